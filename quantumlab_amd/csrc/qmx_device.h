@@ -1,0 +1,110 @@
+// Device-side helpers shared by the qmx_*.hip kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "qmx_internal.h"
+
+namespace qmx {
+
+// amplitude storage types: one 16 B (double) or 8 B (float) vector per amplitude
+template <typename T> struct Amp;
+template <> struct alignas(16) Amp<double> { double x, y; };
+template <> struct alignas(8) Amp<float> { float x, y; };
+
+constexpr int kBlock = 256;   // 4 wave64 per workgroup
+
+// run a statement with T bound to the register's storage type
+#define QMX_DISPATCH(prec, T, ...)                  \
+    do {                                            \
+        if ((prec) == FP64) {                       \
+            using T = double;                       \
+            __VA_ARGS__;                            \
+        } else {                                    \
+            using T = float;                        \
+            __VA_ARGS__;                            \
+        }                                           \
+    } while (0)
+
+
+#define QMX_HIP_CHECK(call)                                                            \
+    do {                                                                               \
+        hipError_t err_ = (call);                                                      \
+        if (err_ != hipSuccess) ::qmx::hipFatal(err_, #call, __FILE__, __LINE__);      \
+    } while (0)
+
+[[noreturn]] void hipFatal(hipError_t err, const char *what, const char *file, int line);
+
+inline hipStream_t hstream() { return reinterpret_cast<hipStream_t>(stream()); }
+
+// insert a zero bit at position p of x
+__device__ __forceinline__ uint64_t insertZero(uint64_t x, int p) {
+    uint64_t lo = x & ((1ull << p) - 1ull);
+    return ((x >> p) << (p + 1)) | lo;
+}
+
+// insert zero bits at the ascending positions of a BitList
+__device__ __forceinline__ uint64_t insertZeros(uint64_t x, const BitList &b) {
+    for (int j = 0; j < b.n; ++j) x = insertZero(x, b.pos[j]);
+    return x;
+}
+
+// parity of popcount
+__device__ __forceinline__ int parity64(uint64_t x) { return __popcll(x) & 1; }
+
+// Neumaier-compensated FP64 accumulator
+struct Acc {
+    double s, c;
+    __device__ __forceinline__ Acc() : s(0.0), c(0.0) {}
+    __device__ __forceinline__ void add(double v) {
+        double t = s + v;
+        if (fabs(s) >= fabs(v)) c += (s - t) + v;
+        else c += (v - t) + s;
+        s = t;
+    }
+    __device__ __forceinline__ void merge(double os, double oc) {
+        add(os);
+        c += oc;
+    }
+    __device__ __forceinline__ double value() const { return s + c; }
+};
+
+// Reduce NV compensated accumulators across the workgroup (kBlock threads).
+// Thread 0 returns the block totals in out[]; deterministic order.
+template <int NV>
+__device__ __forceinline__ void blockReduce(Acc (&acc)[NV], double (&out)[NV]) {
+    // wave64 butterfly on (s, c) pairs
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            double os = __shfl_xor(acc[v].s, off, 64);
+            double oc = __shfl_xor(acc[v].c, off, 64);
+            acc[v].merge(os, oc);
+        }
+    }
+    __shared__ double lds[kBlock / 64][NV][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { lds[wave][v][0] = acc[v].s; lds[wave][v][1] = acc[v].c; }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            Acc a;
+            for (int w = 0; w < kBlock / 64; ++w) a.merge(lds[w][v][0], lds[w][v][1]);
+            out[v] = a.value();
+        }
+    }
+}
+
+inline unsigned gridFor(long long work, int perThread) {
+    long long per = (long long)kBlock * perThread;
+    long long g = (work + per - 1) / per;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+}  // namespace qmx

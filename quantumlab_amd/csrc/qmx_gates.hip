@@ -1,0 +1,424 @@
+// Gate kernels: every gate is one streaming sweep over exactly the amplitudes
+// it changes.
+//
+// Index scheme: a launch enumerates work items w in [0, N / 2^b), where b is
+// the number of "fixed" bit positions (target + controls, or the tile bits of
+// applyMatrixN). Each w is expanded to a state index by inserting zeros at
+// those positions (ascending) and then OR-ing the control mask, so controlled
+// gates touch only the 2^(n-c) amplitudes whose controls are 1 instead of
+// launching N/2 threads and idling most of them (QuEST_gpu.cu:1239-1308).
+// Consecutive lanes get consecutive w, so every load/store instruction of a
+// wave64 covers contiguous memory for all bits below the lowest fixed bit.
+#include <algorithm>
+#include <vector>
+
+#include "qmx_device.h"
+
+namespace qmx {
+
+namespace {
+
+template <typename T> struct C2 { T re, im; };
+
+template <typename T>
+__device__ __forceinline__ Amp<T> cmul2(const C2<T> &u0, const Amp<T> &a, const C2<T> &u1,
+                                        const Amp<T> &b) {
+    // u0*a + u1*b
+    T re = u0.re * a.x - u0.im * a.y + u1.re * b.x - u1.im * b.y;
+    T im = u0.re * a.y + u0.im * a.x + u1.re * b.y + u1.im * b.x;
+    return Amp<T>{re, im};
+}
+
+template <typename T> struct M2 { C2<T> m00, m01, m10, m11; };
+
+constexpr int kPairsPerThread = 4;
+
+// ---- dense 2x2 on one target (optionally controlled) ----------------------
+// QuEST_cpu.c:1737-1794 (compactUnitary), :2020-2062 (unitary), :2251-2317
+template <typename T, bool CTRL>
+__global__ void __launch_bounds__(kBlock) kMatrix2(Amp<T> *__restrict__ a, long long work,
+                                                   int target, BitList fixed, uint64_t setMask,
+                                                   M2<T> u) {
+    const uint64_t half = 1ull << target;
+    long long w = (long long)blockIdx.x * kBlock * kPairsPerThread + threadIdx.x;
+    uint64_t i0[kPairsPerThread];
+    Amp<T> up[kPairsPerThread], lo[kPairsPerThread];
+    bool ok[kPairsPerThread];
+#pragma unroll
+    for (int j = 0; j < kPairsPerThread; ++j) {
+        long long wj = w + (long long)j * kBlock;
+        ok[j] = wj < work;
+        uint64_t idx = CTRL ? (insertZeros((uint64_t)wj, fixed) | setMask) : insertZero((uint64_t)wj, target);
+        i0[j] = idx;
+        if (ok[j]) {
+            up[j] = a[idx];
+            lo[j] = a[idx | half];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kPairsPerThread; ++j) {
+        if (ok[j]) {
+            Amp<T> nu = cmul2(u.m00, up[j], u.m01, lo[j]);
+            Amp<T> nl = cmul2(u.m10, up[j], u.m11, lo[j]);
+            a[i0[j]] = nu;
+            a[i0[j] | half] = nl;
+        }
+    }
+}
+
+// ---- permutations ----------------------------------------------------------
+// multiControlledMultiQubitNot (QuEST_cpu.c multiControlledMultiQubitNotLocal),
+// controlledNot (QuEST_cpu.c:2662-2710): swap i <-> i ^ targMask
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kNot(Amp<T> *__restrict__ a, long long work, BitList fixed,
+                                               uint64_t setMask, uint64_t targMask) {
+    long long w = (long long)blockIdx.x * kBlock * kPairsPerThread + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kPairsPerThread; ++j) {
+        long long wj = w + (long long)j * kBlock;
+        if (wj < work) {
+            uint64_t i = insertZeros((uint64_t)wj, fixed) | setMask;
+            uint64_t k = i ^ targMask;
+            Amp<T> x = a[i], y = a[k];
+            a[i] = y;
+            a[k] = x;
+        }
+    }
+}
+
+// swapGate (QuEST_cpu.c swapQubitAmpsLocal): |..0..1..> <-> |..1..0..>
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kSwap(Amp<T> *__restrict__ a, long long work, BitList fixed,
+                                                uint64_t m1, uint64_t m2) {
+    long long w = (long long)blockIdx.x * kBlock * kPairsPerThread + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kPairsPerThread; ++j) {
+        long long wj = w + (long long)j * kBlock;
+        if (wj < work) {
+            uint64_t base = insertZeros((uint64_t)wj, fixed);
+            uint64_t i = base | m2, k = base | m1;
+            Amp<T> x = a[i], y = a[k];
+            a[i] = y;
+            a[k] = x;
+        }
+    }
+}
+
+// ---- diagonal gates --------------------------------------------------------
+// phaseShiftByTerm / multiControlledPhaseShift / phase flips: amp *= term on
+// the 2^(n-m) amplitudes whose mask bits are all 1
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kPhaseAllOnes(Amp<T> *__restrict__ a, long long work,
+                                                        BitList fixed, uint64_t mask, C2<T> t) {
+    long long w = (long long)blockIdx.x * kBlock * kPairsPerThread + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kPairsPerThread; ++j) {
+        long long wj = w + (long long)j * kBlock;
+        if (wj < work) {
+            uint64_t i = insertZeros((uint64_t)wj, fixed) | mask;
+            Amp<T> v = a[i];
+            a[i] = Amp<T>{t.re * v.x - t.im * v.y, t.re * v.y + t.im * v.x};
+        }
+    }
+}
+
+// multiRotateZ (QuEST_cpu.c:3299-3339): amp *= cos(θ/2) - i·f·sin(θ/2),
+// f = +1 / -1 for even / odd parity of (i & targMask); only where ctrls are 1
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kRotZ(Amp<T> *__restrict__ a, long long work, BitList fixed,
+                                                uint64_t setMask, uint64_t targMask, T c, T s) {
+    long long w = (long long)blockIdx.x * kBlock * kPairsPerThread + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kPairsPerThread; ++j) {
+        long long wj = w + (long long)j * kBlock;
+        if (wj < work) {
+            uint64_t i = insertZeros((uint64_t)wj, fixed) | setMask;
+            T f = parity64(i & targMask) ? -s : s;
+            Amp<T> v = a[i];
+            a[i] = Amp<T>{c * v.x + f * v.y, c * v.y - f * v.x};
+        }
+    }
+}
+
+// ---- Pauli-string rotation in one sweep -------------------------------------
+// exp(-iθ/2 P) with P = i^nY X^x Z^z (Y = iXZ): out = cos·ψ - i·sin·(Pψ),
+// (Pψ)_j = i^nY (-1)^{|(j^x)&z|} ψ_{j^x}. Replaces QuEST's basis change +
+// multiRotateZ + undo (QuEST_common.c:411-448), i.e. 1 sweep instead of 1+2w.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kPauliRot(Amp<T> *__restrict__ a, long long work,
+                                                    BitList fixed, uint64_t setMask, uint64_t xMask,
+                                                    uint64_t zMask, C2<T> ph, T c, T s) {
+    // ph = i^nY ; coefficient on the partner: -i·s·ph·(±1) = k·(±1)
+    const T kre = s * ph.im, kim = -s * ph.re;  // -i*s*ph
+    long long w = (long long)blockIdx.x * kBlock * kPairsPerThread + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kPairsPerThread; ++j) {
+        long long wj = w + (long long)j * kBlock;
+        if (wj < work) {
+            uint64_t i = insertZeros((uint64_t)wj, fixed) | setMask;
+            uint64_t p = i ^ xMask;
+            Amp<T> vi = a[i], vp = a[p];
+            // out_i uses sign of (p & z), out_p uses sign of (i & z)
+            T si = parity64(p & zMask) ? (T)-1 : (T)1;
+            T sp = parity64(i & zMask) ? (T)-1 : (T)1;
+            Amp<T> oi{c * vi.x + si * (kre * vp.x - kim * vp.y), c * vi.y + si * (kre * vp.y + kim * vp.x)};
+            Amp<T> op{c * vp.x + sp * (kre * vi.x - kim * vi.y), c * vp.y + sp * (kre * vi.y + kim * vi.x)};
+            a[i] = oi;
+            a[p] = op;
+        }
+    }
+}
+
+// ---- dense 2^k x 2^k on k targets: LDS-tiled --------------------------------
+// QuEST_cpu.c:1895-2007 semantics (targs[0] = least significant bit of the
+// matrix row/column index). A workgroup owns a tile of 2^(k+L) amplitudes:
+// all target combinations times the L lowest non-target bits, with the tile's
+// bit set S sorted so that loading tile index t from base | deposit(t, S)
+// walks memory in increasing order (contiguous runs for the low bits).
+// Matrix and tile live in LDS; no per-call global scratch (the reference
+// allocated 2^k·grid·24 B of global memory per call, QuEST_gpu.cu:1065-1073).
+struct TileSpec {
+    int k;            // targets
+    int tileBits;     // k + L
+    int sPos[32];     // global bit position of each tile bit (ascending)
+    int tLocal[16];   // tile-local position of targs[j]
+    int gLocal[16];   // tile-local position of group bit j
+    BitList outer;    // bits not in the tile and not controls (for block base), ascending (as fixed list)
+    uint64_t ctrlMask;
+};
+
+template <typename T, int MAXK>
+__global__ void __launch_bounds__(kBlock) kMatrixNTile(Amp<T> *__restrict__ a, long long numTiles,
+                                                       TileSpec ts, const C2<T> *__restrict__ gU,
+                                                       int matInLds) {
+    extern __shared__ unsigned char smem[];
+    const int k = ts.k, dim = 1 << k;
+    const int tileAmps = 1 << ts.tileBits;
+    const int L = ts.tileBits - k, G = 1 << L;
+    Amp<T> *tile = reinterpret_cast<Amp<T> *>(smem);
+    C2<T> *mat = reinterpret_cast<C2<T> *>(smem + sizeof(Amp<T>) * tileAmps);
+    int *cOff = reinterpret_cast<int *>(smem + sizeof(Amp<T>) * tileAmps +
+                                        (matInLds ? sizeof(C2<T>) * dim * dim : 0));
+
+    if (matInLds)
+        for (int e = threadIdx.x; e < dim * dim; e += kBlock) mat[e] = gU[e];
+    for (int c = threadIdx.x; c < dim; c += kBlock) {
+        int off = 0;
+        for (int j = 0; j < k; ++j)
+            if ((c >> j) & 1) off |= 1 << ts.tLocal[j];
+        cOff[c] = off;
+    }
+    const C2<T> *U = matInLds ? mat : gU;
+
+    for (long long tileId = blockIdx.x; tileId < numTiles; tileId += gridDim.x) {
+        const uint64_t base = insertZeros((uint64_t)tileId, ts.outer) | ts.ctrlMask;
+        __syncthreads();
+        for (int t = threadIdx.x; t < tileAmps; t += kBlock) {
+            uint64_t idx = base;
+            for (int b = 0; b < ts.tileBits; ++b)
+                if ((t >> b) & 1) idx |= 1ull << ts.sPos[b];
+            tile[t] = a[idx];
+        }
+        __syncthreads();
+        // each thread computes outputs o = tid + m*kBlock; r = o >> L, g = o & (G-1)
+        constexpr int kMaxOut = 4;  // tiles hold at most 4 * kBlock amplitudes
+        Amp<T> res[kMaxOut];
+        int nOut = 0;
+        for (int o = threadIdx.x; o < tileAmps && nOut < kMaxOut; o += kBlock, ++nOut) {
+            int r = o >> L, g = o & (G - 1);
+            int gOff = 0;
+            for (int j = 0; j < L; ++j)
+                if ((g >> j) & 1) gOff |= 1 << ts.gLocal[j];
+            T re = 0, im = 0;
+            const C2<T> *row = U + (size_t)r * dim;
+            for (int c = 0; c < dim; ++c) {
+                C2<T> m = row[c];
+                Amp<T> v = tile[gOff | cOff[c]];
+                re += m.re * v.x - m.im * v.y;
+                im += m.re * v.y + m.im * v.x;
+            }
+            res[nOut] = Amp<T>{re, im};
+        }
+        __syncthreads();
+        nOut = 0;
+        for (int o = threadIdx.x; o < tileAmps && nOut < kMaxOut; o += kBlock, ++nOut) {
+            int r = o >> L, g = o & (G - 1);
+            int gOff = 0;
+            for (int j = 0; j < L; ++j)
+                if ((g >> j) & 1) gOff |= 1 << ts.gLocal[j];
+            tile[gOff | cOff[r]] = res[nOut];
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < tileAmps; t += kBlock) {
+            uint64_t idx = base;
+            for (int b = 0; b < ts.tileBits; ++b)
+                if ((t >> b) & 1) idx |= 1ull << ts.sPos[b];
+            a[idx] = tile[t];
+        }
+    }
+}
+
+BitList makeBits(uint64_t mask) {
+    BitList b;
+    b.n = 0;
+    for (int p = 0; p < 64; ++p)
+        if ((mask >> p) & 1ull) b.pos[b.n++] = p;
+    return b;
+}
+
+template <typename T> C2<T> c2(const Cplx &z) { return C2<T>{(T)z.re, (T)z.im}; }
+
+// device copy of a dense matrix, cached by content size (grown on demand)
+void *g_matBuf = nullptr;
+size_t g_matBytes = 0;
+
+}  // namespace
+
+void applyMatrix2(QuregDev *q, int target, uint64_t ctrlMask, uint64_t flipMask, const Cplx u[4]) {
+    const uint64_t tbit = 1ull << target;
+    const int c = __builtin_popcountll(ctrlMask);
+    const long long work = q->numAmps >> (1 + c);
+    const uint64_t setMask = ctrlMask & ~flipMask;
+    BitList fixed = makeBits(ctrlMask | tbit);
+    QMX_DISPATCH(q->prec, T, {
+        M2<T> m{c2<T>(u[0]), c2<T>(u[1]), c2<T>(u[2]), c2<T>(u[3])};
+        if (ctrlMask)
+            hipLaunchKernelGGL((kMatrix2<T, true>), dim3(gridFor(work, kPairsPerThread)), dim3(kBlock),
+                               0, hstream(), (Amp<T> *)q->amps, work, target, fixed, setMask, m);
+        else
+            hipLaunchKernelGGL((kMatrix2<T, false>), dim3(gridFor(work, kPairsPerThread)), dim3(kBlock),
+                               0, hstream(), (Amp<T> *)q->amps, work, target, fixed, setMask, m);
+    });
+    checkLaunch("applyMatrix2");
+}
+
+void applyNot(QuregDev *q, uint64_t ctrlMask, uint64_t targMask) {
+    const uint64_t lowT = targMask & (~targMask + 1ull);
+    const int c = __builtin_popcountll(ctrlMask);
+    const long long work = q->numAmps >> (1 + c);
+    BitList fixed = makeBits(ctrlMask | lowT);
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kNot<T>, dim3(gridFor(work, kPairsPerThread)), dim3(kBlock), 0, hstream(),
+                           (Amp<T> *)q->amps, work, fixed, ctrlMask, targMask);
+    });
+    checkLaunch("applyNot");
+}
+
+void applySwap(QuregDev *q, int q1, int q2) {
+    const uint64_t m1 = 1ull << q1, m2 = 1ull << q2;
+    const long long work = q->numAmps >> 2;
+    BitList fixed = makeBits(m1 | m2);
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kSwap<T>, dim3(gridFor(work, kPairsPerThread)), dim3(kBlock), 0, hstream(),
+                           (Amp<T> *)q->amps, work, fixed, m1, m2);
+    });
+    checkLaunch("applySwap");
+}
+
+void applyPhaseOnAllOnes(QuregDev *q, uint64_t mask, Cplx term) {
+    const int m = __builtin_popcountll(mask);
+    const long long work = q->numAmps >> m;
+    BitList fixed = makeBits(mask);
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kPhaseAllOnes<T>, dim3(gridFor(work, kPairsPerThread)), dim3(kBlock), 0,
+                           hstream(), (Amp<T> *)q->amps, work, fixed, mask, c2<T>(term));
+    });
+    checkLaunch("applyPhaseOnAllOnes");
+}
+
+void applyMultiRotateZ(QuregDev *q, uint64_t ctrlMask, uint64_t targMask, double angle) {
+    const int c = __builtin_popcountll(ctrlMask);
+    const long long work = q->numAmps >> c;
+    BitList fixed = makeBits(ctrlMask);
+    const double cs = cos(angle / 2.0), sn = sin(angle / 2.0);
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kRotZ<T>, dim3(gridFor(work, kPairsPerThread)), dim3(kBlock), 0, hstream(),
+                           (Amp<T> *)q->amps, work, fixed, ctrlMask, targMask, (T)cs, (T)sn);
+    });
+    checkLaunch("applyMultiRotateZ");
+}
+
+void applyPauliRotation(QuregDev *q, uint64_t ctrlMask, uint64_t xMask, uint64_t zMask, int numY,
+                        double angle) {
+    if (xMask == 0) {
+        // diagonal: Z-string rotation (numY == 0 here since Y sets x)
+        applyMultiRotateZ(q, ctrlMask, zMask, angle);
+        return;
+    }
+    const uint64_t lowX = xMask & (~xMask + 1ull);
+    const int c = __builtin_popcountll(ctrlMask);
+    const long long work = q->numAmps >> (1 + c);
+    BitList fixed = makeBits(ctrlMask | lowX);
+    const double cs = cos(angle / 2.0), sn = sin(angle / 2.0);
+    // i^numY
+    static const double phr[4] = {1, 0, -1, 0}, phi[4] = {0, 1, 0, -1};
+    Cplx ph{phr[numY & 3], phi[numY & 3]};
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kPauliRot<T>, dim3(gridFor(work, kPairsPerThread)), dim3(kBlock), 0,
+                           hstream(), (Amp<T> *)q->amps, work, fixed, ctrlMask, xMask, zMask,
+                           c2<T>(ph), (T)cs, (T)sn);
+    });
+    checkLaunch("applyPauliRotation");
+}
+
+void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const Cplx *u) {
+    const int n = q->numQubits;
+    const int c = __builtin_popcountll(ctrlMask);
+    uint64_t tmask = 0;
+    for (int j = 0; j < k; ++j) tmask |= 1ull << targs[j];
+    // choose L low non-target, non-control bits so a tile has >= 256 amps (>= 1 per lane)
+    int free = n - k - c;
+    int L = std::max(0, std::min(free, std::max(0, 8 - k)));
+    uint64_t gmask = 0;
+    for (int p = 0, got = 0; p < n && got < L; ++p) {
+        uint64_t b = 1ull << p;
+        if (!(tmask & b) && !(ctrlMask & b)) { gmask |= b; ++got; }
+    }
+    TileSpec ts;
+    ts.k = k;
+    ts.tileBits = k + L;
+    uint64_t smask = tmask | gmask;
+    int nb = 0;
+    for (int p = 0; p < 64; ++p)
+        if ((smask >> p) & 1ull) ts.sPos[nb++] = p;
+    auto localOf = [&](int pos) {
+        for (int b = 0; b < nb; ++b)
+            if (ts.sPos[b] == pos) return b;
+        return -1;
+    };
+    for (int j = 0; j < k; ++j) ts.tLocal[j] = localOf(targs[j]);
+    for (int j = 0, p = 0; p < 64 && j < L; ++p)
+        if ((gmask >> p) & 1ull) ts.gLocal[j++] = localOf(p);
+    ts.outer = makeBits(smask | ctrlMask);
+    ts.ctrlMask = ctrlMask;
+    const long long numTiles = q->numAmps >> (k + L + c);
+    const int dim = 1 << k;
+    const int tileAmps = 1 << (k + L);
+    if (k > 10) {
+        std::fprintf(stderr, "libquest_mi355x: applyMatrixN supports at most 10 targets (got %d)\n", k);
+        std::abort();
+    }
+
+    QMX_DISPATCH(q->prec, T, {
+        size_t matBytes = sizeof(C2<T>) * (size_t)dim * dim;
+        std::vector<C2<T>> hm((size_t)dim * dim);
+        for (size_t e = 0; e < hm.size(); ++e) hm[e] = c2<T>(u[e]);
+        if (g_matBytes < matBytes) {
+            devFree(g_matBuf);
+            g_matBuf = devAlloc(matBytes);
+            g_matBytes = matBytes;
+        }
+        QMX_HIP_CHECK(hipMemcpyAsync(g_matBuf, hm.data(), matBytes, hipMemcpyHostToDevice, hstream()));
+        size_t tileBytes = sizeof(Amp<T>) * tileAmps;
+        int matInLds = (tileBytes + matBytes + dim * sizeof(int) <= 64 * 1024) ? 1 : 0;
+        size_t shm = tileBytes + (matInLds ? matBytes : 0) + dim * sizeof(int);
+        long long grid = std::min<long long>(numTiles, 65536);
+        hipLaunchKernelGGL((kMatrixNTile<T, 16>), dim3((unsigned)grid), dim3(kBlock), shm, hstream(),
+                           (Amp<T> *)q->amps, numTiles, ts, (const C2<T> *)g_matBuf, matInLds);
+        // the host vector must outlive the async copy
+        QMX_HIP_CHECK(hipStreamSynchronize(hstream()));
+    });
+    checkLaunch("applyMatrixN");
+}
+
+}  // namespace qmx

@@ -1,0 +1,117 @@
+// Internal interface between the QuEST C-API front-end (quest_api.cpp) and the
+// HIP backend (qmx_*.hip). Nothing here is exported.
+//
+// Device data layout (one register):
+//   amps   2^numQubitsInStateVec complex amplitudes, interleaved (re, im),
+//          stored as double2 (QMX_PREC_FP64) or float2 (QMX_PREC_FP32).
+//          Interleaving makes every amplitude one 16 B (or 8 B) vector load,
+//          so a 1-target gate streams two arrays instead of the four SoA
+//          streams of the reference GPU backend (QuEST_gpu.cu:783-844).
+//   The host mirror of the Qureg stays SoA double, as callers expect.
+#pragma once
+
+#include <cstdint>
+#include <cstddef>
+
+#include "QuEST.h"
+
+namespace qmx {
+
+enum Prec : int { FP64 = 0, FP32 = 1 };
+
+struct Cplx {
+    double re, im;
+};
+
+// per-register record; Qureg.firstLevelReduction points at it
+struct QuregDev {
+    uint32_t magic;          // kMagic while alive
+    int prec;                // Prec
+    int numQubits;           // qubits in the state vector (2n for density matrices)
+    int isDensity;
+    int numQubitsRep;        // qubits represented
+    long long numAmps;
+    void *amps;              // device amplitudes
+    size_t hostBytes;        // bytes of each host mirror array (mmap'd)
+};
+constexpr uint32_t kMagic = 0x514d5833u;  // "QMX3"
+
+inline QuregDev *dev(const Qureg &q) { return reinterpret_cast<QuregDev *>(q.firstLevelReduction); }
+inline size_t ampBytes(int prec) { return prec == FP64 ? 16 : 8; }
+
+// maximum bit positions a kernel accepts in one index-insertion list
+constexpr int kMaxBits = 64;
+
+struct BitList {
+    int n;
+    int pos[kMaxBits];  // ascending
+};
+
+// ---- runtime (qmx_runtime.hip) ---------------------------------------------
+void runtimeInit();                 // device check + stream; idempotent
+void *stream();                     // hipStream_t of the library
+void deviceSync();
+void *devAlloc(size_t bytes);       // fails loudly
+void devFree(void *p);
+void hostToDevice(void *dst, const void *src, size_t bytes);
+void deviceToHost(void *dst, const void *src, size_t bytes);
+double *pinnedScratch();            // 64 doubles of pinned host memory
+void checkLaunch(const char *what);
+void timerBegin();
+double timerEndMs();
+int deviceDescription(char *buf, int len);
+
+// ---- state set-up (qmx_state.hip) ------------------------------------------
+void initBlank(QuregDev *q);
+void initClassical(QuregDev *q, long long index);          // statevec index
+void initUniform(QuregDev *q, double re);                  // every amp = re (plus states)
+void initDebug(QuregDev *q);
+void initSingleQubit(QuregDev *q, int qubit, int outcome);
+void initPureDensity(QuregDev *rho, const QuregDev *psi);  // rho = |psi><psi|
+void copyState(QuregDev *dst, const QuregDev *src);        // any precisions
+void uploadAmps(QuregDev *q, long long start, const double *re, const double *im, long long count);
+void downloadAmps(const QuregDev *q, long long start, double *re, double *im, long long count);
+void weightedSum(Cplx f1, const QuregDev *q1, Cplx f2, const QuregDev *q2, Cplx fo, QuregDev *out);
+void mixDensityMatrix(QuregDev *combine, double otherProb, const QuregDev *other);
+void scaleAll(QuregDev *q, double factor);
+
+// ---- gates (qmx_gates.hip) -------------------------------------------------
+// u = [[u00,u01],[u10,u11]] on `target`, applied where (i & ctrlMask) ^ flipMask == ctrlMask
+void applyMatrix2(QuregDev *q, int target, uint64_t ctrlMask, uint64_t flipMask, const Cplx u[4]);
+// amplitude i <-> i ^ targMask for every i whose ctrl bits are all 1
+void applyNot(QuregDev *q, uint64_t ctrlMask, uint64_t targMask);
+void applySwap(QuregDev *q, int q1, int q2);
+// amp *= term where (i & mask) == mask (phase shift family)
+void applyPhaseOnAllOnes(QuregDev *q, uint64_t mask, Cplx term);
+// amp *= exp(-i angle/2 * (+1/-1 by parity of i & targMask)) where ctrl bits are 1
+void applyMultiRotateZ(QuregDev *q, uint64_t ctrlMask, uint64_t targMask, double angle);
+// dense 2^k x 2^k matrix (row-major, u[r*dim+c]) on targets (targs[0] = least
+// significant bit of the row index), applied where ctrl bits are 1
+void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const Cplx *u);
+// exp(-i angle/2 * P), P = X^{xMask} Z^{zMask} with Y = iXZ folded in (single sweep)
+void applyPauliRotation(QuregDev *q, uint64_t ctrlMask, uint64_t xMask, uint64_t zMask,
+                        int numY, double angle);
+
+// ---- reductions (qmx_reduce.hip); all accumulate in FP64 -------------------
+double probOfZero(const QuregDev *q, int qubit);                   // state vector
+double totalProb(const QuregDev *q);                               // state vector
+Cplx innerProduct(const QuregDev *bra, const QuregDev *ket);
+double densProbOfZero(const QuregDev *q, int qubit);               // Σ Re ρ_ii, bit = 0
+double densTotalProb(const QuregDev *q);
+double purity(const QuregDev *q);                                  // Σ |ρ_ij|^2
+double densInnerProduct(const QuregDev *a, const QuregDev *b);     // Σ Re(a* b)
+double hsDistanceSquared(const QuregDev *a, const QuregDev *b);
+double densFidelity(const QuregDev *rho, const QuregDev *psi);     // <psi|rho|psi>
+void probOfAllOutcomes(const QuregDev *q, const int *qubits, int k, double *out);
+double normSquaredThenScale(QuregDev *q, bool rescale);            // FP64 norm, optional 1/sqrt renorm
+double maxAbsDiff(const QuregDev *a, const QuregDev *b);           // max_i |a_i - b_i|
+
+// ---- non-unitary (qmx_density.hip) -----------------------------------------
+void collapse(QuregDev *q, int qubit, int outcome, double prob);
+void densCollapse(QuregDev *q, int qubit, int outcome, double prob);
+void densDephase(QuregDev *q, int qubit, double retain);
+void densTwoQubitDephase(QuregDev *q, int q1, int q2, double retain);
+void densDepolarise(QuregDev *q, int qubit, double depolLevel);
+void densDamp(QuregDev *q, int qubit, double damping);
+
+}  // namespace qmx

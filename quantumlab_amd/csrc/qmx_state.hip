@@ -1,0 +1,268 @@
+// State set-up, host-mirror transfer and element-wise combination kernels.
+// Each is one coalesced streaming sweep over the interleaved amplitudes.
+#include <algorithm>
+#include <mutex>
+
+#include "qmx_device.h"
+
+namespace qmx {
+
+namespace {
+
+template <typename T>
+__global__ void kFill(Amp<T> *a, long long n, T re, T im) {
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) a[i] = Amp<T>{re, im};
+}
+
+template <typename T>
+__global__ void kSetOne(Amp<T> *a, long long idx, T re, T im) {
+    a[idx] = Amp<T>{re, im};
+}
+
+// QuEST_cpu.c:1655-1660: amp k = (2k + i(2k+1)) / 10
+template <typename T>
+__global__ void kDebug(Amp<T> *a, long long n) {
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) a[i] = Amp<T>{(T)((i * 2.0) / 10.0), (T)((i * 2.0 + 1.0) / 10.0)};
+}
+
+// QuEST_cpu.c:1617-1628: qubit fixed to `outcome`, others in |+>
+template <typename T>
+__global__ void kSingleQubit(Amp<T> *a, long long n, int qubit, int outcome, double norm) {
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) {
+            int bit = (int)((i >> qubit) & 1);
+            a[i] = Amp<T>{bit == outcome ? (T)norm : (T)0, (T)0};
+        }
+}
+
+// rho[r + c*dim] = psi[r] * conj(psi[c]) (QuEST_cpu.c:1216-1253)
+template <typename TR, typename TP>
+__global__ void kPureDensity(Amp<TR> *rho, const Amp<TP> *psi, int nRep) {
+    long long dim = 1ll << nRep;
+    long long n = dim * dim;
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) {
+            long long r = i & (dim - 1), c = i >> nRep;
+            double kr = psi[r].x, ki = psi[r].y;
+            double br = psi[c].x, bi = -(double)psi[c].y;
+            rho[i] = Amp<TR>{(TR)(kr * br - ki * bi), (TR)(kr * bi + ki * br)};
+        }
+}
+
+template <typename TD, typename TS>
+__global__ void kCopy(Amp<TD> *d, const Amp<TS> *s, long long n) {
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) d[i] = Amp<TD>{(TD)s[i].x, (TD)s[i].y};
+}
+
+template <typename T>
+__global__ void kInterleave(Amp<T> *a, const double *re, const double *im, long long n) {
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) a[i] = Amp<T>{(T)re[i], (T)im[i]};
+}
+
+template <typename T>
+__global__ void kDeinterleave(const Amp<T> *a, double *re, double *im, long long n) {
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) {
+            Amp<T> v = a[i];
+            re[i] = (double)v.x;
+            im[i] = (double)v.y;
+        }
+}
+
+// out = fac1*q1 + fac2*q2 + facOut*out (QuEST_cpu.c statevec_setWeightedQureg)
+template <typename T>
+__global__ void kWeighted(Cplx f1, const Amp<T> *a, Cplx f2, const Amp<T> *b, Cplx fo, Amp<T> *o,
+                          long long n) {
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) {
+            double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y, orr = o[i].x, oi = o[i].y;
+            double re = fo.re * orr - fo.im * oi + f1.re * ar - f1.im * ai + f2.re * br - f2.im * bi;
+            double im = fo.re * oi + fo.im * orr + f1.re * ai + f1.im * ar + f2.re * bi + f2.im * br;
+            o[i] = Amp<T>{(T)re, (T)im};
+        }
+}
+
+// combine = (1-p) combine + p other (QuEST_cpu.c:908-936)
+template <typename T>
+__global__ void kMixDens(Amp<T> *c, const Amp<T> *o, double p, long long n) {
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) {
+            double cr = c[i].x * (1 - p), ci = c[i].y * (1 - p);
+            cr += p * o[i].x;
+            ci += p * o[i].y;
+            c[i] = Amp<T>{(T)cr, (T)ci};
+        }
+}
+
+template <typename T>
+__global__ void kScale(Amp<T> *a, long long n, double f) {
+    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < 4; ++j, i += kBlock)
+        if (i < n) a[i] = Amp<T>{(T)(a[i].x * f), (T)(a[i].y * f)};
+}
+
+// staging buffer for SoA host <-> interleaved device transfers
+constexpr long long kStageAmps = 1ll << 22;
+std::mutex g_stageMu;
+double *g_stage = nullptr;  // 2 * kStageAmps doubles
+
+double *stage() {
+    if (!g_stage) g_stage = static_cast<double *>(devAlloc(2 * kStageAmps * sizeof(double)));
+    return g_stage;
+}
+
+}  // namespace
+
+void initBlank(QuregDev *q) {
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kFill<T>, dim3(gridFor(q->numAmps, 4)), dim3(kBlock), 0, hstream(),
+                           (Amp<T> *)q->amps, q->numAmps, (T)0, (T)0);
+    });
+    checkLaunch("initBlank");
+}
+
+void initClassical(QuregDev *q, long long index) {
+    initBlank(q);
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kSetOne<T>, dim3(1), dim3(1), 0, hstream(), (Amp<T> *)q->amps, index,
+                           (T)1, (T)0);
+    });
+    checkLaunch("initClassical");
+}
+
+void initUniform(QuregDev *q, double re) {
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kFill<T>, dim3(gridFor(q->numAmps, 4)), dim3(kBlock), 0, hstream(),
+                           (Amp<T> *)q->amps, q->numAmps, (T)re, (T)0);
+    });
+    checkLaunch("initUniform");
+}
+
+void initDebug(QuregDev *q) {
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kDebug<T>, dim3(gridFor(q->numAmps, 4)), dim3(kBlock), 0, hstream(),
+                           (Amp<T> *)q->amps, q->numAmps);
+    });
+    checkLaunch("initDebug");
+}
+
+void initSingleQubit(QuregDev *q, int qubit, int outcome) {
+    double norm = 1.0 / sqrt((double)q->numAmps / 2.0);
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kSingleQubit<T>, dim3(gridFor(q->numAmps, 4)), dim3(kBlock), 0,
+                           hstream(), (Amp<T> *)q->amps, q->numAmps, qubit, outcome, norm);
+    });
+    checkLaunch("initSingleQubit");
+}
+
+void initPureDensity(QuregDev *rho, const QuregDev *psi) {
+    QMX_DISPATCH(rho->prec, TR, {
+        QMX_DISPATCH(psi->prec, TP, {
+            hipLaunchKernelGGL((kPureDensity<TR, TP>), dim3(gridFor(rho->numAmps, 4)),
+                               dim3(kBlock), 0, hstream(), (Amp<TR> *)rho->amps,
+                               (const Amp<TP> *)psi->amps, rho->numQubitsRep);
+        });
+    });
+    checkLaunch("initPureDensity");
+}
+
+void copyState(QuregDev *dst, const QuregDev *src) {
+    if (dst->prec == src->prec) {
+        QMX_HIP_CHECK(hipMemcpyAsync(dst->amps, src->amps, src->numAmps * ampBytes(src->prec),
+                                     hipMemcpyDeviceToDevice, hstream()));
+        return;
+    }
+    QMX_DISPATCH(dst->prec, TD, {
+        QMX_DISPATCH(src->prec, TS, {
+            hipLaunchKernelGGL((kCopy<TD, TS>), dim3(gridFor(src->numAmps, 4)), dim3(kBlock), 0,
+                               hstream(), (Amp<TD> *)dst->amps, (const Amp<TS> *)src->amps,
+                               src->numAmps);
+        });
+    });
+    checkLaunch("copyState");
+}
+
+void uploadAmps(QuregDev *q, long long start, const double *re, const double *im, long long count) {
+    std::lock_guard<std::mutex> lk(g_stageMu);
+    double *s = stage();
+    for (long long off = 0; off < count; off += kStageAmps) {
+        long long m = std::min(kStageAmps, count - off);
+        QMX_HIP_CHECK(hipMemcpyAsync(s, re + off, m * sizeof(double), hipMemcpyHostToDevice, hstream()));
+        QMX_HIP_CHECK(hipMemcpyAsync(s + kStageAmps, im + off, m * sizeof(double),
+                                     hipMemcpyHostToDevice, hstream()));
+        QMX_DISPATCH(q->prec, T, {
+            hipLaunchKernelGGL(kInterleave<T>, dim3(gridFor(m, 4)), dim3(kBlock), 0, hstream(),
+                               (Amp<T> *)q->amps + start + off, s, s + kStageAmps, m);
+        });
+        checkLaunch("uploadAmps");
+    }
+    QMX_HIP_CHECK(hipStreamSynchronize(hstream()));
+}
+
+void downloadAmps(const QuregDev *q, long long start, double *re, double *im, long long count) {
+    std::lock_guard<std::mutex> lk(g_stageMu);
+    double *s = stage();
+    for (long long off = 0; off < count; off += kStageAmps) {
+        long long m = std::min(kStageAmps, count - off);
+        QMX_DISPATCH(q->prec, T, {
+            hipLaunchKernelGGL(kDeinterleave<T>, dim3(gridFor(m, 4)), dim3(kBlock), 0, hstream(),
+                               (const Amp<T> *)q->amps + start + off, s, s + kStageAmps, m);
+        });
+        checkLaunch("downloadAmps");
+        QMX_HIP_CHECK(hipMemcpyAsync(re + off, s, m * sizeof(double), hipMemcpyDeviceToHost, hstream()));
+        QMX_HIP_CHECK(hipMemcpyAsync(im + off, s + kStageAmps, m * sizeof(double),
+                                     hipMemcpyDeviceToHost, hstream()));
+        QMX_HIP_CHECK(hipStreamSynchronize(hstream()));
+    }
+}
+
+void weightedSum(Cplx f1, const QuregDev *q1, Cplx f2, const QuregDev *q2, Cplx fo, QuregDev *out) {
+    QMX_DISPATCH(out->prec, T, {
+        hipLaunchKernelGGL(kWeighted<T>, dim3(gridFor(out->numAmps, 4)), dim3(kBlock), 0,
+                           hstream(), f1, (const Amp<T> *)q1->amps, f2, (const Amp<T> *)q2->amps,
+                           fo, (Amp<T> *)out->amps, out->numAmps);
+    });
+    checkLaunch("weightedSum");
+}
+
+void mixDensityMatrix(QuregDev *combine, double otherProb, const QuregDev *other) {
+    QMX_DISPATCH(combine->prec, T, {
+        hipLaunchKernelGGL(kMixDens<T>, dim3(gridFor(combine->numAmps, 4)), dim3(kBlock), 0,
+                           hstream(), (Amp<T> *)combine->amps, (const Amp<T> *)other->amps,
+                           otherProb, combine->numAmps);
+    });
+    checkLaunch("mixDensityMatrix");
+}
+
+void scaleAll(QuregDev *q, double factor) {
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kScale<T>, dim3(gridFor(q->numAmps, 4)), dim3(kBlock), 0, hstream(),
+                           (Amp<T> *)q->amps, q->numAmps, factor);
+    });
+    checkLaunch("scaleAll");
+}
+
+}  // namespace qmx
