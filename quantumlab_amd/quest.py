@@ -192,6 +192,37 @@ _SIGS = {
     "mixPauli": (None, [Qureg, C.c_int, qreal, qreal, qreal]),
     "mixKrausMap": (None, [Qureg, C.c_int, C.POINTER(ComplexMatrix2), C.c_int]),
     "mixDensityMatrix": (None, [Qureg, qreal, Qureg]),
+    "mixTwoQubitDepolarising": (None, [Qureg, C.c_int, C.c_int, qreal]),
+    "mixTwoQubitKrausMap": (None, [Qureg, C.c_int, C.c_int, C.POINTER(ComplexMatrix4), C.c_int]),
+    "mixMultiQubitKrausMap": (None, [Qureg, C.POINTER(C.c_int), C.c_int, C.POINTER(ComplexMatrixN), C.c_int]),
+    "mixNonTPKrausMap": (None, [Qureg, C.c_int, C.POINTER(ComplexMatrix2), C.c_int]),
+    "calcExpecPauliProd": (qreal, [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, Qureg]),
+    "calcExpecPauliSum": (qreal, [Qureg, C.POINTER(C.c_int), C.POINTER(qreal), C.c_int, Qureg]),
+    "calcExpecPauliHamil": (qreal, [Qureg, PauliHamil, Qureg]),
+    "applyPauliSum": (None, [Qureg, C.POINTER(C.c_int), C.POINTER(qreal), C.c_int, Qureg]),
+    "applyPauliHamil": (None, [Qureg, PauliHamil, Qureg]),
+    "setWeightedQureg": (None, [Complex, Qureg, Complex, Qureg, Complex, Qureg]),
+    "sqrtSwapGate": (None, [Qureg, C.c_int, C.c_int]),
+    "controlledTwoQubitUnitary": (None, [Qureg, C.c_int, C.c_int, C.c_int, ComplexMatrix4]),
+    "multiControlledTwoQubitUnitary": (None, [Qureg, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int,
+                                              ComplexMatrix4]),
+    "applyMatrix2": (None, [Qureg, C.c_int, ComplexMatrix2]),
+    "applyMatrix4": (None, [Qureg, C.c_int, C.c_int, ComplexMatrix4]),
+    "applyMatrixN": (None, [Qureg, C.POINTER(C.c_int), C.c_int, ComplexMatrixN]),
+    "applyProjector": (None, [Qureg, C.c_int, C.c_int]),
+    "getProbAmp": (qreal, [Qureg, C.c_longlong]),
+    "getDensityAmp": (Complex, [Qureg, C.c_longlong, C.c_longlong]),
+    "getNumQubits": (C.c_int, [Qureg]),
+    "getNumAmps": (C.c_longlong, [Qureg]),
+    "setAmps": (None, [Qureg, C.c_longlong, C.POINTER(qreal), C.POINTER(qreal), C.c_longlong]),
+    "setDensityAmps": (None, [Qureg, C.c_longlong, C.c_longlong, C.POINTER(qreal), C.POINTER(qreal),
+                              C.c_longlong]),
+    "multiControlledMultiRotateZ": (None, [Qureg, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int), C.c_int,
+                                           qreal]),
+    "multiControlledMultiRotatePauli": (None, [Qureg, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int), C.c_int, qreal]),
+    "controlledRotateAroundAxis": (None, [Qureg, C.c_int, C.c_int, qreal, Vector]),
+    "controlledPauliY": (None, [Qureg, C.c_int, C.c_int]),
 }
 
 # extension entry points of libquest_mi355x (include/quest_mi355x.h)
