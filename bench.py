@@ -197,7 +197,9 @@ def main():
     if not args.no_fused:
         q32, wall32, blocks, norm2 = run_fp32_fused(qe, n, plans, args.warmup, args.steps, args.fuse_qubits)
         ip = qe.calcInnerProduct(q64, q32)
-        fid = ip.real ** 2 + ip.imag ** 2
+        # normalised fidelity: norms from FP64 reductions (the FP32-stored vector
+        # re-rounds its norm^2 by ~1e-9 even right after renormalisation)
+        fid = (ip.real ** 2 + ip.imag ** 2) / (qe.calcTotalProb(q64) * qe.calcTotalProb(q32))
         maxabs = qe.qmxMaxAbsDiff(q64, q32)
         p_err = max(abs(qe.calcProbOfOutcome(q64, k, 0) - qe.calcProbOfOutcome(q32, k, 0)) for k in range(0, n, 4))
         result["fused_fp32"] = {"speedup_vs_fp64_per_gate": round(wall / wall32, 3),

@@ -51,6 +51,7 @@ struct Block {
     std::vector<int> gates;     // indices into the gate list, in order
     std::vector<int> targs;     // ascending qubits (targs[0] = LSB of the block matrix)
     std::vector<Cplx> u;        // dense block matrix, row-major
+    void *prepared5 = nullptr;  // device-resident form for the FP32 MFMA kernel
 };
 
 Cplx cmul(Cplx a, Cplx b) { return Cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
@@ -121,6 +122,11 @@ struct qmxCircuit {
     std::vector<Gate> gates;
     std::vector<Block> blocks;
     int fusedQubits = 0;
+    void releaseBlocks() {
+        for (Block &b : blocks) qmx::fused5Free(b.prepared5);
+        blocks.clear();
+    }
+    ~qmxCircuit() { releaseBlocks(); }
 };
 
 namespace {
@@ -215,7 +221,7 @@ void qmxCircuitMultiQubitUnitary(qmxCircuit *c, int *targs, int numTargs, const 
 }
 
 int qmxCircuitFuse(qmxCircuit *c, int maxQubits) {
-    c->blocks.clear();
+    c->releaseBlocks();
     c->fusedQubits = maxQubits;
     for (int gi = 0; gi < (int)c->gates.size(); ++gi) {
         const uint64_t q = c->gates[gi].qubitMask();
@@ -245,6 +251,16 @@ int qmxCircuitFuse(qmxCircuit *c, int maxQubits) {
             c->blocks[best].gates.push_back(gi);
         }
     }
+    // The MFMA kernel applies exactly-5-qubit blocks: pad smaller multi-gate
+    // blocks with identity on extra qubits, taken from the top so the low
+    // qubits stay non-targets (consecutive groups = consecutive addresses).
+    if (maxQubits == 5 && c->numQubits >= 9) {
+        for (Block &b : c->blocks) {
+            if (b.gates.size() < 2) continue;
+            for (int qb = c->numQubits - 1; qb >= 0 && __builtin_popcountll(b.mask) < 5; --qb)
+                b.mask |= 1ull << qb;
+        }
+    }
     for (Block &b : c->blocks) buildBlockMatrix(*c, b);
     return (int)c->blocks.size();
 }
@@ -257,9 +273,14 @@ void qmxCircuitApply(Qureg qureg, qmxCircuit *c, int fused) {
         for (const Gate &g : c->gates) applyGate(d, g);
         return;
     }
-    for (const Block &b : c->blocks) {
+    for (Block &b : c->blocks) {
         if (b.gates.size() == 1) {
             applyGate(d, c->gates[b.gates[0]]);  // a lone gate keeps its cheaper kernel
+            continue;
+        }
+        if (d->prec == qmx::FP32 && b.targs.size() == 5 && d->numQubits >= 9) {
+            if (!b.prepared5) b.prepared5 = qmx::fused5Prepare(b.targs.data(), b.u.data());
+            qmx::fused5Apply(d, b.prepared5);
             continue;
         }
         qmx::applyMatrixN(d, 0, b.targs.data(), (int)b.targs.size(), b.u.data());

@@ -38,6 +38,47 @@ constexpr int kBlock = 256;   // 4 wave64 per workgroup
 
 inline hipStream_t hstream() { return reinterpret_cast<hipStream_t>(stream()); }
 
+// 16 B (double) / 8 B (float) vector view of one amplitude, for single-instruction
+// loads and stores, optionally nontemporal. Nontemporal streaming raised the
+// 2x2 pair sweep at 32 qubits from 5.5 to 6.2-6.6 TB/s on MI355X when every
+// wave's contiguous runs are >= 128 B; shorter runs (pair distance < 8
+// amplitudes) are faster with ordinary accesses.
+template <typename T> struct VecOf;
+template <> struct VecOf<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct VecOf<float> { typedef float type __attribute__((ext_vector_type(2))); };
+
+template <typename T, bool NT>
+__device__ __forceinline__ Amp<T> ldA(const Amp<T> *p) {
+    typedef typename VecOf<T>::type V;
+    V v = NT ? __builtin_nontemporal_load(reinterpret_cast<const V *>(p)) : *reinterpret_cast<const V *>(p);
+    return Amp<T>{v.x, v.y};
+}
+
+template <typename T, bool NT>
+__device__ __forceinline__ void stA(Amp<T> *p, const Amp<T> &a) {
+    typedef typename VecOf<T>::type V;
+    V v;
+    v.x = a.x;
+    v.y = a.y;
+    if (NT) __builtin_nontemporal_store(v, reinterpret_cast<V *>(p));
+    else *reinterpret_cast<V *>(p) = v;
+}
+
+// XCD-aware workgroup order. MI355X dispatches consecutive workgroups
+// round-robin over its 8 XCDs; remapping so that XCD x (= blockIdx % 8) owns
+// the x-th contiguous eighth of the work lets each XCD stream one region.
+// Measured +5-7% on streaming sweeps at 32 qubits.
+__device__ __forceinline__ long long xcdBlock() {
+    const long long b = blockIdx.x, nb = gridDim.x;
+    if (nb & 7) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// nontemporal accesses pay off once a wave's contiguous runs reach 128 B
+inline bool useNT(int lowestFixedBit, int prec) {
+    return ((1ll << lowestFixedBit) * (prec == FP64 ? 16 : 8)) >= 128;
+}
+
 // insert a zero bit at position p of x
 __device__ __forceinline__ uint64_t insertZero(uint64_t x, int p) {
     uint64_t lo = x & ((1ull << p) - 1ull);

@@ -1,0 +1,141 @@
+// Fused dense 5-qubit blocks on FP32 amplitudes, as a GEMM on MFMA.
+//
+// A block U (32x32 complex) acting on 5 target qubits maps every group of 32
+// amplitudes (one per target combination, the other qubits fixed) to U times
+// that group. Written on real and imaginary parts, a column x = [Re; Im] of
+// 64 floats goes to M x with M = [[Ur, -Ui], [Ui, Ur]] (64x64). For 16 groups
+// at once that is OUT(64x16) = M(64x64) * X(64x16): 4 output tiles of 16x16,
+// each the sum of 16 v_mfma_f32_16x16x4f32 steps over K = 64.
+//
+// Operand placement (16x16x4 f32 MFMA, wave64):
+//   A (16x4): lane l holds A[l%16][l/16]  -> M fragments, 64 floats per lane,
+//             loaded once per wave and kept in VGPRs for all its groups;
+//   B (4x16): lane l holds B[l/16][l%16]  -> lane l loads amplitude
+//             (group l%16, combination 4s + l/16) for s = 0..7: its .x feeds
+//             K-slice s (real rows), its .y K-slice s+8 (imaginary rows);
+//   D (16x16): lane l holds D[4(l/16)+v][l%16], v = 0..3.
+// So amplitudes go from HBM straight into MFMA operands and back, with no LDS
+// staging; 16 lanes of a wave read 16 consecutive groups, i.e. consecutive
+// addresses whenever the lowest qubit is not a target.
+//
+// Cost per amplitude: 256 FP32 MFMA flops for 16 B of HBM traffic, which on
+// MI355X (157.3 TFLOP/s FP32 matrix, 8 TB/s HBM) keeps the block near the
+// memory bound; every fused block replaces ~10 per-gate sweeps.
+#include <algorithm>
+#include <vector>
+
+#include "qmx_device.h"
+
+namespace qmx {
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+struct Fused5Spec {
+    BitList targetsSorted;   // 5 ascending target positions (for group -> base index)
+    uint64_t off[32];        // offset of combination c: sum_j bit_j(c) << targs[j]
+};
+
+constexpr int kChunksPerWave = 8;  // 16-group chunks handled by one wave
+
+__global__ void __launch_bounds__(kBlock) kFused5(Amp<float> *__restrict__ a, long long numChunks,
+                                                  Fused5Spec spec, const float *__restrict__ M) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    // A fragments: Af[rt][s] = M[16 rt + col][4 s + kq]
+    float Af[4][16];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int s = 0; s < 16; ++s) Af[rt][s] = M[(16 * rt + col) * 64 + 4 * s + kq];
+
+    const long long firstChunk = (xcdBlock() * (kBlock / 64) + wave) * kChunksPerWave;
+    for (int it = 0; it < kChunksPerWave; ++it) {
+        const long long chunk = firstChunk + it;
+        if (chunk >= numChunks) break;
+        const uint64_t g = (uint64_t)chunk * 16 + col;
+        const uint64_t base = insertZeros(g, spec.targetsSorted);
+        Amp<float> x[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) x[s] = ldA<float, true>(a + (base + spec.off[4 * s + kq]));
+        f4 acc[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            acc[rt] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[rt][s], x[s].x, acc[rt], 0, 0, 0);
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[rt][s + 8], x[s].y, acc[rt], 0, 0, 0);
+        }
+        // lane holds output rows 4 kq + v of each tile: combination c = 16 h + 4 kq + v,
+        // real part from tile h, imaginary part from tile h + 2
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = 16 * h + 4 * kq + v;
+                stA<float, true>(a + (base + spec.off[c]), Amp<float>{acc[h][v], acc[h + 2][v]});
+            }
+    }
+}
+
+struct Fused5Prepared {
+    Fused5Spec spec;
+    float *M;  // device copy of the 64x64 real block matrix
+};
+
+}  // namespace
+
+// Build the device-resident form of a 5-qubit block once (targs[0] = least
+// significant bit of u's index, u row-major 32x32).
+void *fused5Prepare(const int *targs, const Cplx *u) {
+    auto *p = new Fused5Prepared;
+    uint64_t tmask = 0;
+    for (int j = 0; j < 5; ++j) tmask |= 1ull << targs[j];
+    p->spec.targetsSorted.n = 0;
+    for (int b = 0; b < 64; ++b)
+        if ((tmask >> b) & 1ull) p->spec.targetsSorted.pos[p->spec.targetsSorted.n++] = b;
+    for (int c = 0; c < 32; ++c) {
+        uint64_t o = 0;
+        for (int j = 0; j < 5; ++j)
+            if ((c >> j) & 1) o |= 1ull << targs[j];
+        p->spec.off[c] = o;
+    }
+    std::vector<float> hm(64 * 64);
+    for (int r = 0; r < 32; ++r)
+        for (int c = 0; c < 32; ++c) {
+            const Cplx z = u[r * 32 + c];
+            hm[r * 64 + c] = (float)z.re;
+            hm[r * 64 + 32 + c] = (float)-z.im;
+            hm[(32 + r) * 64 + c] = (float)z.im;
+            hm[(32 + r) * 64 + 32 + c] = (float)z.re;
+        }
+    p->M = static_cast<float *>(devAlloc(hm.size() * sizeof(float)));
+    hostToDevice(p->M, hm.data(), hm.size() * sizeof(float));
+    return p;
+}
+
+void fused5Free(void *prepared) {
+    auto *p = static_cast<Fused5Prepared *>(prepared);
+    if (!p) return;
+    devFree(p->M);
+    delete p;
+}
+
+// Apply a prepared block to an FP32 register of >= 9 qubits (one 16-group
+// chunk is 512 amplitudes).
+void fused5Apply(QuregDev *q, const void *prepared) {
+    const auto *p = static_cast<const Fused5Prepared *>(prepared);
+    const long long numChunks = (q->numAmps >> 5) / 16;
+    const long long wavesNeeded = (numChunks + kChunksPerWave - 1) / kChunksPerWave;
+    const long long grid = (wavesNeeded + (kBlock / 64) - 1) / (kBlock / 64);
+    hipLaunchKernelGGL(kFused5, dim3((unsigned)grid), dim3(kBlock), 0, hstream(), (Amp<float> *)q->amps,
+                       numChunks, p->spec, (const float *)p->M);
+    checkLaunch("fused5Apply");
+}
+
+}  // namespace qmx

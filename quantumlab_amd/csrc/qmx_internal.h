@@ -88,6 +88,11 @@ void applyMultiRotateZ(QuregDev *q, uint64_t ctrlMask, uint64_t targMask, double
 // dense 2^k x 2^k matrix (row-major, u[r*dim+c]) on targets (targs[0] = least
 // significant bit of the row index), applied where ctrl bits are 1
 void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const Cplx *u);
+// dense 5-qubit block on an FP32 register via MFMA (qmx_fused.hip); prepared
+// once (device-resident matrix), applied to registers of >= 9 qubits
+void *fused5Prepare(const int *targs, const Cplx *u);
+void fused5Free(void *prepared);
+void fused5Apply(QuregDev *q, const void *prepared);
 // exp(-i angle/2 * P), P = X^{xMask} Z^{zMask} with Y = iXZ folded in (single sweep)
 void applyPauliRotation(QuregDev *q, uint64_t ctrlMask, uint64_t xMask, uint64_t zMask,
                         int numY, double angle);

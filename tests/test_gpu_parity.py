@@ -266,6 +266,60 @@ def test_fused_blocks_equal_per_gate(gpu, k):
     gpu.qmxCircuitDestroy(c)
 
 
+def _fidelity_after_renorm(gpu, q64, q32):
+    """1 - |<a|b>|^2 / (|a|^2 |b|^2), norms in FP64. The normalised form is used
+    because storing the renormalised vector in FP32 rounds its norm^2 again by
+    ~1e-9, which is representation noise, not simulation error."""
+    gpu.qmxRenormalise(q32, 1)
+    ip = gpu.calcInnerProduct(q64, q32)
+    f = (ip.real ** 2 + ip.imag ** 2) / (gpu.calcTotalProb(q64) * gpu.calcTotalProb(q32))
+    return 1 - f, gpu.qmxMaxAbsDiff(q64, q32)
+
+
+@pytest.mark.parametrize("n", [9, 12, 20])
+def test_fp32_mfma_five_qubit_unitary_matches_fp64(gpu, n):
+    """multiQubitUnitary with 5 targets on an FP32 register runs the MFMA kernel;
+    targets unsorted and including qubit 0 and the top qubit."""
+    rng = np.random.default_rng(n)
+    psi = random_state(n, 40 + n)
+    q64, q32 = gpu.qureg(n), gpu.qureg(n, precision=QMX_PREC_FP32)
+    gpu.set_state(q64, psi)
+    gpu.set_state(q32, psi)
+    for targs in ([n - 1, 0, 3, 2, 5], [4, 1, 7, 0, 8], [int(x) for x in rng.choice(n, 5, replace=False)]):
+        u = W.haar_unitary(rng, 32)
+        gpu.multiQubitUnitary(q64, targs, u)
+        gpu.multiQubitUnitary(q32, targs, u)
+    one_minus_f, maxabs = _fidelity_after_renorm(gpu, q64, q32)
+    scale = np.max(np.abs(gpu.state(q64)))
+    assert maxabs < 2e-6 * scale
+    assert one_minus_f < 1e-9
+    gpu.destroy(q64)
+    gpu.destroy(q32)
+
+
+def test_fused_fp32_mfma_circuit_matches_fp64(gpu):
+    """Brick circuit fused into 5-qubit MFMA blocks on FP32 amplitudes against
+    the per-gate FP64 run: fidelity error <= 1e-6, max-abs
+    amplitude error at FP32 resolution."""
+    n = 18
+    gates = W.brick_circuit(n, 15, seed=13, extra_every=5)
+    c = W.build_circuit(gpu, n, gates)
+    nb = gpu.qmxCircuitFuse(c, 5)
+    assert nb < len(gates) / 4
+    q64, q32 = gpu.qureg(n), gpu.qureg(n, precision=QMX_PREC_FP32)
+    W.apply_gates(gpu, q64, gates)
+    gpu.qmxCircuitApply(q32, c, 1)
+    one_minus_f, maxabs = _fidelity_after_renorm(gpu, q64, q32)
+    assert one_minus_f < 1e-6
+    assert maxabs < 1e-5 * np.max(np.abs(gpu.state(q64)))
+    p64 = [gpu.calcProbOfOutcome(q64, k, 0) for k in range(n)]
+    p32 = [gpu.calcProbOfOutcome(q32, k, 0) for k in range(n)]
+    assert np.max(np.abs(np.array(p64) - np.array(p32))) < 1e-6
+    gpu.destroy(q64)
+    gpu.destroy(q32)
+    gpu.qmxCircuitDestroy(c)
+
+
 # ---- input validation (QuEST340/tests/*: message substrings) ------------------------------------
 
 @pytest.mark.parametrize("call,substr", [
