@@ -71,6 +71,11 @@ def run_fp64_per_gate(qe, n, steps, warmup, seed):
     qe.qmxSync()
     wall = time.perf_counter() - t0
     qe.check()
+    # FP64 norm check + renormalisation (outside the timed region): one reduction
+    # and one in-place scale sweep of known size (read + write 16 B per amplitude),
+    # which also calibrates the HBM traffic counters against a plain stream
+    norm2 = qe.qmxRenormalise(q, 1)
+    log(f"fp64 norm^2 after timed steps {norm2:.15f}")
     cu = [(qe.lib.qmxTimerSlotMs(sl), t) for sl, t in slots]
     timed = [g for s in range(warmup, warmup + steps) for g in plans[s]]
     log(f"fp64 timed {steps} steps: wall {wall:.3f}s events {ev_ms / 1e3:.3f}s")
@@ -163,6 +168,15 @@ def main():
     for m, t in cu:
         per_target.setdefault(t, []).append(m)
     tmeans = {t: float(np.mean(v)) for t, v in per_target.items()}
+    # HBM bytes per launch of the dominant kernel, from the calibrated PMC runs
+    # summarised in profiles/brick32_fp64_pmc_traffic.md (same workload)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "brick32_fp64_traffic.json")
+    if os.path.exists(tpath):
+        tj = json.load(open(tpath))
+        key = f"kMatrix2<double, false, true>|{N // 2}"
+        if tj.get("qubits") == n and key in tj["bytes_per_dispatch"]:
+            traffic = tj["bytes_per_dispatch"][key]
 
     result = {
         "metric": METRIC,
@@ -184,9 +198,10 @@ def main():
         "gates_per_s": round(len(timed) / wall, 2),
         "roofline": {"bound": "hbm", "kernel": "kMatrix2<double> (compactUnitary 2x2 pair sweep)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(float(cu_ms.mean()), 4), "launches": int(len(cu_ms)),
                      "algorithmic_bytes_per_launch": cu_bytes,
+                     "traffic_source": "profiles/brick32_fp64_pmc_traffic.md" if traffic else None,
                      "per_target_ms_min": round(min(tmeans.values()), 4),
                      "per_target_ms_max": round(max(tmeans.values()), 4)},
         "event_time_s": round(ev_ms / 1e3, 4),
