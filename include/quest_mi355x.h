@@ -74,6 +74,35 @@ int qmxCircuitNumBlocks(const qmxCircuit *c);
  * must have been called), otherwise one QuEST-equivalent kernel per gate */
 void qmxCircuitApply(Qureg qureg, qmxCircuit *c, int fused);
 
+/* ---- batched swap tests (quantum k-means) ------------------------------------ */
+/* A plan holds numPoints real vectors (point p has 2^pointQubits[p] entries,
+ * packed back to back in `points`) and numCentroids centroids of 2^maxQubits
+ * entries each. Every point is amplitude-encoded (uniformly controlled Ry
+ * tree) into a q-qubit register and swap-tested against every centroid
+ * truncated to the point's 2^q entries and L2-normalised, on a (2q+1)-qubit
+ * register; P(ancilla = 0) = (1 + |<a|b>|^2)/2. One k-means iteration assigns
+ * each point to the centroid of largest P0 and moves each centroid to the mean
+ * of its points (coordinate j over the points that have it); it runs as one
+ * persistent swap-test launch plus four small kernels, optionally replayed as
+ * a captured HIP graph. precision: QMX_PREC_FP64 or QMX_PREC_FP32 (amplitudes;
+ * sums are FP64 either way). Returns NULL on invalid sizes. */
+typedef struct qmxSwapPlan qmxSwapPlan;
+qmxSwapPlan *qmxSwapPlanCreate(int numPoints, const int *pointQubits, const double *points, int numCentroids,
+                               int maxQubits, const double *centroids, int precision);
+void qmxSwapPlanDestroy(qmxSwapPlan *plan);
+/* times candidate tile shapes per register size on the device, keeps the fastest;
+ * returns the number of timings made */
+int qmxSwapPlanAutotune(qmxSwapPlan *plan);
+int qmxSwapPlanRowsPerTile(const qmxSwapPlan *plan, int qubits);
+int qmxSwapPlanNumTiles(const qmxSwapPlan *plan);
+void qmxSwapPlanRun(qmxSwapPlan *plan, int iterations, int useGraph);
+/* P(ancilla = 0) of the last iteration's tests, numPoints x numCentroids */
+void qmxSwapPlanP0(qmxSwapPlan *plan, double *out);
+void qmxSwapPlanAssignments(qmxSwapPlan *plan, int *out);
+void qmxSwapPlanCentroids(qmxSwapPlan *plan, double *out);
+/* the encoded register state of point p (2^q amplitudes) */
+void qmxSwapPlanPointState(qmxSwapPlan *plan, int p, double *out);
+
 #ifdef __cplusplus
 }
 #endif

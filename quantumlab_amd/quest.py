@@ -255,6 +255,17 @@ _EXT_SIGS = {
     "qmxCircuitFuse": (C.c_int, [C.c_void_p, C.c_int]),
     "qmxCircuitNumBlocks": (C.c_int, [C.c_void_p]),
     "qmxCircuitApply": (None, [Qureg, C.c_void_p, C.c_int]),
+    "qmxSwapPlanCreate": (C.c_void_p, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int, C.c_int,
+                                       C.POINTER(C.c_double), C.c_int]),
+    "qmxSwapPlanDestroy": (None, [C.c_void_p]),
+    "qmxSwapPlanAutotune": (C.c_int, [C.c_void_p]),
+    "qmxSwapPlanRowsPerTile": (C.c_int, [C.c_void_p, C.c_int]),
+    "qmxSwapPlanNumTiles": (C.c_int, [C.c_void_p]),
+    "qmxSwapPlanRun": (None, [C.c_void_p, C.c_int, C.c_int]),
+    "qmxSwapPlanP0": (None, [C.c_void_p, C.POINTER(C.c_double)]),
+    "qmxSwapPlanAssignments": (None, [C.c_void_p, C.POINTER(C.c_int)]),
+    "qmxSwapPlanCentroids": (None, [C.c_void_p, C.POINTER(C.c_double)]),
+    "qmxSwapPlanPointState": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "qmxMtInitByArray": (None, [C.POINTER(C.c_ulong), C.c_int]),
     "qmxMtNextU32": (C.c_uint, []),
     "qmxMtNextReal1": (C.c_double, []),
