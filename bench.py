@@ -18,6 +18,14 @@ speed-up and fidelity / max-abs error against the FP64 run; and the CPU
 baseline: QuEST 3.4's own OpenMP CPU backend (oracle/_ref, compiled from the
 reference sources) or, where that is absent, the oracle's C restatement,
 timed on this host on a bounded sample.
+
+`swap_tests`: the batched QKMeans workload (16384 points of dimension 2^q,
+q uniform in 4..10, 16 centroids, 10 k-means iterations; amplitude encoding,
+swap test of every point against every centroid, assignment and centroid
+update per iteration, FP32 registers and FP64 sums, one HIP graph per
+iteration) in swap-tests/s, with its distance error against exact FP64 inner
+products, assignment agreement with the FP64 run, and its own CPU baseline:
+the reference's gate-by-gate swap-test circuits on QuEST's CPU backend.
 """
 from __future__ import annotations
 
@@ -130,6 +138,124 @@ def cpu_baseline(n_cpu, seed):
             "gates_per_s": round(len(gates) / dt, 2)}
 
 
+def exact_p0(data, cents):
+    """P(ancilla = 0) = (1 + <a|b>^2)/2 from FP64 numpy inner products of the
+    normalised points and the normalised truncated centroids."""
+    import numpy as np
+    out = np.empty((len(data.qubits), cents.shape[0]))
+    for q in np.unique(data.qubits):
+        idx = np.nonzero(data.qubits == q)[0]
+        D = 1 << int(q)
+        A = np.stack([data.point(p) for p in idx])
+        A /= np.linalg.norm(A, axis=1, keepdims=True)
+        B = cents[:, :D]
+        nb = np.linalg.norm(B, axis=1)
+        B = B / np.where(nb > 0, nb, 1.0)[:, None]
+        out[idx] = 0.5 * (1.0 + (A @ B.T) ** 2)
+    return out
+
+
+def run_swap_tests(qe, points, centroids, iterations, warmup, seed):
+    """Batched QKMeans: amplitude encoding + swap tests of every point against
+    every centroid, `iterations` k-means iterations, FP32 registers with FP64
+    sums, one captured graph per iteration. Checked iteration by iteration
+    against the FP64 plan (assignments) and exact FP64 inner products
+    (distances), then timed from the same initial centroids."""
+    import numpy as np
+    from quantumlab_amd import qkmeans as K
+    from quantumlab_amd.quest import QMX_PREC_FP32, QMX_PREC_FP64
+    data = K.make_dataset(num_points=points, num_clusters=centroids, q_lo=4, q_hi=10, seed=seed)
+    tests = points * centroids
+    p64 = K.SwapPlan(qe, data, QMX_PREC_FP64)
+    p32 = K.SwapPlan(qe, data, QMX_PREC_FP32)
+    same_assign, dist_err, dist_err64 = True, 0.0, 0.0
+    for it in range(iterations):
+        cents = p64.centroids()
+        exact = exact_p0(data, cents)
+        p64.run(1, graph=False)
+        p32.run(1, graph=False)
+        a64, a32 = p64.assignments(), p32.assignments()
+        same_assign &= bool(np.array_equal(a64, a32))
+        # distance the swap test reports: <a|b>^2 = 2 P0 - 1
+        dist_err = max(dist_err, float(np.max(np.abs(2 * p32.p0() - 2 * exact))))
+        dist_err64 = max(dist_err64, float(np.max(np.abs(2 * p64.p0() - 2 * exact))))
+    final64 = (p64.assignments(), p64.p0())
+    p32.close()
+    plan = K.SwapPlan(qe, data, QMX_PREC_FP32)
+    timings = plan.autotune()
+    rows = {q: plan.rows_per_tile(q) for q in range(4, 11)}
+    plan.run(warmup, graph=True)
+    plan.set_centroids(data.init)
+    qe.qmxSync()
+    qe.qmxTimerBegin()
+    t0 = time.perf_counter()
+    plan.run(iterations, graph=True)
+    qe.qmxSync()
+    wall = time.perf_counter() - t0
+    ev_ms = qe.qmxTimerEndMs()
+    timed_same = bool(np.array_equal(plan.assignments(), final64[0]))
+    timed_err = float(np.max(np.abs(2 * plan.p0() - 2 * final64[1])))
+    evals = K.swap_test_amplitude_evaluations(data)
+    res = {"swap_tests_per_s": round(tests * iterations / wall, 1), "unit": "swap-tests/s",
+           "workload": f"{points} points x {centroids} centroids, q in 4..10 (registers 2q+1 = 9..21 qubits), "
+                       f"{iterations} k-means iterations, FP32 amplitudes + FP64 sums, one HIP graph per iteration",
+           "ms_per_iteration": round(wall / iterations * 1e3, 3), "event_ms_per_iteration": round(ev_ms / iterations, 3),
+           "tests_per_iteration": tests, "ancilla0_amplitudes_per_iteration": evals,
+           "amplitudes_per_s": evals * iterations / wall,
+           "assignments_identical_to_fp64_every_iteration": same_assign,
+           "max_distance_error_vs_exact_fp64": dist_err, "max_distance_error_fp64_plan": dist_err64,
+           "timed_run_assignments_identical_to_fp64": timed_same, "timed_run_max_distance_diff_vs_fp64": timed_err,
+           "autotune_timings": timings, "rows_per_tile": rows, "tiles": plan.num_tiles()}
+    plan.close()
+    p64.close()
+    return res, data
+
+
+def cpu_swap_baseline(data, seconds_per_size=2.0):
+    """The reference's own method on the CPU: each swap test built gate by gate
+    (CreateSwapTestReal's list: H, Ry/CX encoders, CX + Toffoli CSWAPs, H) and
+    run through QuEST 3.4's CPU backend (oracle/_ref) or the C restatement,
+    one register of 2q+1 qubits per test, P(ancilla = 0) read with
+    calcProbOfOutcome. Sample: per register size q = 4..10 (the workload draws q
+    uniformly), tests until `seconds_per_size` is spent (3 to 200 tests);
+    rate = 1 / mean over q of the per-test time."""
+    import numpy as np
+    from oracle import swaptest_oracle as O
+    from quantumlab_amd.quest import QuEST
+    ref = os.path.join(ROOT, "oracle", "_ref", "libquest_ref.so")
+    port = os.path.join(ROOT, "oracle", "build", "libcpuref.so")
+    kind, path = ("reference", ref) if os.path.exists(ref) else ("port", port)
+    lib = QuEST(path)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    per_q, count, gates_total, err = {}, {}, 0, 0.0
+    for q in range(4, 11):
+        times = []
+        for k, p in enumerate(np.nonzero(data.qubits == q)[0]):
+            if len(times) >= 3 and (sum(times) >= seconds_per_size or len(times) >= 200):
+                break
+            a, b = data.point(int(p)), data.init[k % data.init.shape[0], :1 << q]
+            gates = O.swap_test_gates(a, b)
+            reg = lib.qureg(2 * q + 1)
+            t0 = time.perf_counter()
+            O.run_gates_quest(lib, reg, gates)
+            p0 = lib.lib.calcProbOfOutcome(reg, 0, 0)
+            times.append(time.perf_counter() - t0)
+            lib.destroy(reg)
+            gates_total += len(gates)
+            ov = (a @ b) / (np.linalg.norm(a) * np.linalg.norm(b))
+            err = max(err, abs(p0 - 0.5 * (1 + ov ** 2)))
+        per_q[q] = float(np.mean(times))
+        count[q] = len(times)
+    mean_s = float(np.mean(list(per_q.values())))
+    return {"value": round(1.0 / mean_s, 3), "unit": "swap-tests/s", "cores": threads, "kind": kind,
+            "sample": f"{sum(count.values())} swap tests, {min(count.values())} to {max(count.values())} per q = 4..10 ({gates_total} QuEST calls in all) on {2 * 4 + 1}..{2 * 10 + 1}-qubit "
+                      f"FP64 registers, gate by gate as the reference builds them; "
+                      f"{'QuEST 3.4 CPU backend compiled from the reference sources' if kind == 'reference' else 'C restatement of it'}, "
+                      f"OpenMP {threads} threads; {sum(per_q[q] * count[q] for q in per_q):.2f} s",
+            "seconds_per_test_by_q": {q: round(t, 4) for q, t in per_q.items()},
+            "max_p0_error_vs_closed_form": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +267,10 @@ def main():
     ap.add_argument("--cpu-qubits", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
+    ap.add_argument("--no-swap", action="store_true")
+    ap.add_argument("--swap-points", type=int, default=16384)
+    ap.add_argument("--swap-centroids", type=int, default=16)
+    ap.add_argument("--swap-iterations", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the state-vector "
@@ -227,10 +357,22 @@ def main():
         qe.destroy(q32)
     qe.destroy(q64)
 
+    swap_data = None
+    if not args.no_swap:
+        result["swap_tests"], swap_data = run_swap_tests(qe, args.swap_points, args.swap_centroids,
+                                                         args.swap_iterations, args.warmup, args.seed)
+        st = result["swap_tests"]
+        log(f"swap tests: {st['swap_tests_per_s']:.4g}/s, {st['ms_per_iteration']} ms/iteration, "
+            f"distance err {st['max_distance_error_vs_exact_fp64']:.2e}, "
+            f"assignments identical {st['assignments_identical_to_fp64_every_iteration']}")
+
     if not args.no_cpu:
         log(f"cpu baseline at {args.cpu_qubits} qubits ...")
         result["cpu_baseline"] = cpu_baseline(args.cpu_qubits, args.seed)
         log(f"cpu baseline: {result['cpu_baseline']['value']} GB/s")
+        if swap_data is not None:
+            result["swap_tests"]["cpu_baseline"] = cpu_swap_baseline(swap_data)
+            log(f"cpu swap-test baseline: {result['swap_tests']['cpu_baseline']['value']} swap-tests/s")
     print(json.dumps(result), flush=True)
 
 

@@ -100,6 +100,9 @@ void qmxSwapPlanRun(qmxSwapPlan *plan, int iterations, int useGraph);
 void qmxSwapPlanP0(qmxSwapPlan *plan, double *out);
 void qmxSwapPlanAssignments(qmxSwapPlan *plan, int *out);
 void qmxSwapPlanCentroids(qmxSwapPlan *plan, double *out);
+/* replaces the centroids (numCentroids x 2^maxQubits) and re-encodes them;
+ * tiles, tuned shapes and a captured graph are kept */
+void qmxSwapPlanSetCentroids(qmxSwapPlan *plan, const double *centroids);
 /* the encoded register state of point p (2^q amplitudes) */
 void qmxSwapPlanPointState(qmxSwapPlan *plan, int p, double *out);
 

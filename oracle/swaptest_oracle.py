@@ -41,14 +41,6 @@ def _apply_cx(state, n, c, t):
     return out
 
 
-def _apply_ccx(state, n, c1, c2, t):
-    idx = np.arange(state.size)
-    sel = (((idx >> c1) & 1) == 1) & (((idx >> c2) & 1) == 1)
-    out = state.copy()
-    out[sel] = state[idx[sel] ^ (1 << t)]
-    return out
-
-
 def _ry(theta):
     c, s = np.cos(theta / 2), np.sin(theta / 2)
     return np.array([[c, -s], [s, c]])
@@ -103,18 +95,19 @@ def ae_angles(v, eps: float = DBL_EPSILON) -> np.ndarray:
     return Y
 
 
-def ae_circuit_dense(v) -> np.ndarray:
-    """ExchangeToYGate + FRy applied gate by gate to |0...0> of q qubits."""
+def ae_gates(v, qubits) -> list:
+    """ExchangeToYGate (AmplitudeEncodeCuda.cu:498-533) with FRy expanded
+    (FRGate.cu:181-252) as a list of QuEST calls: ("ry", t, theta) and
+    ("cx", c, t). `qubits[j]` is the register qubit of the encoder's qubit j."""
     D = len(v)
     q = D.bit_length() - 1
     Y = ae_angles(v)
-    state = np.zeros(D)
-    state[0] = 1.0
+    gates = []
     idx = D - 2
     for i in range(q):
-        bits = [q - 1 - j for j in range(i + 1)]   # local qubit j -> register qubit bits[j]
+        bits = [qubits[q - 1 - j] for j in range(i + 1)]   # local qubit j -> register qubit bits[j]
         if i == 0:
-            state = _apply_1q(state, q, bits[0], _ry(Y[idx]))
+            gates.append(("ry", bits[0], float(Y[idx])))
             idx -= 2
             continue
         nq = i + 1
@@ -122,11 +115,85 @@ def ae_circuit_dense(v) -> np.ndarray:
         theta = split_angles(Y[idx:idx + deg], deg)
         target = bits[nq - 1]
         for k in range(deg):
-            state = _apply_1q(state, q, target, _ry(theta[k]))
+            gates.append(("ry", target, float(theta[k])))
             ctrl_local = nq - 2 - gray_code_different(k, deg)
-            state = _apply_cx(state, q, bits[ctrl_local], target)
+            gates.append(("cx", bits[ctrl_local], target))
         idx -= 1 << (i + 1)
+    return gates
+
+
+def toffoli_gates(c0, c1, t) -> list:
+    """CreateToffoliGate (CnUGate.h:50-78): H, CX and RZ(-/+ pi/4) only."""
+    T, Td = ("rz", -np.pi / 4), ("rz", np.pi / 4)
+    return [("h", t), ("cx", c1, t), (Td[0], t, Td[1]), ("cx", c0, t), (T[0], t, T[1]), ("cx", c1, t),
+            (Td[0], t, Td[1]), ("cx", c0, t), (T[0], t, T[1]), ("h", t), (T[0], c1, T[1]), ("cx", c0, c1),
+            (T[0], c0, T[1]), (Td[0], c1, Td[1]), ("cx", c0, c1)]
+
+
+def swap_test_gates(v1, v2) -> list:
+    """CreateSwapTestReal (SwapTest.cpp:70-123) on 2q+1 qubits as QuEST calls:
+    H(0); AE(v1) on 1..q; AE(v2) on q+1..2q; per i CSWAP(0, i+1, i+1+q) =
+    CX(i+1+q -> i+1), Toffoli(0, i+1 -> i+1+q), CX(i+1+q -> i+1)
+    (CreateControlledSwap, SimpleGates.cpp:161-176); H(0)."""
+    q = len(v1).bit_length() - 1
+    gates = [("h", 0)]
+    gates += ae_gates(v1, [1 + j for j in range(q)])
+    gates += ae_gates(v2, [1 + q + j for j in range(q)])
+    for i in range(q):
+        t1, t2 = i + 1, i + 1 + q
+        gates += [("cx", t2, t1)] + toffoli_gates(0, t1, t2) + [("cx", t2, t1)]
+    gates.append(("h", 0))
+    return gates
+
+
+def _rz(theta):
+    return np.diag([np.exp(-0.5j * theta), np.exp(0.5j * theta)])
+
+
+_H = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
+
+
+def run_gates_dense(gates, n, state=None) -> np.ndarray:
+    """The gate list on `state` (default |0...0>) of n qubits (dense complex
+    vector, small n)."""
+    if state is None:
+        state = np.zeros(1 << n, dtype=complex)
+        state[0] = 1.0
+    for g in gates:
+        if g[0] == "ry":
+            state = _apply_1q(state, n, g[1], _ry(g[2]))
+        elif g[0] == "rz":
+            state = _apply_1q(state, n, g[1], _rz(g[2]))
+        elif g[0] == "h":
+            state = _apply_1q(state, n, g[1], _H)
+        elif g[0] == "cx":
+            state = _apply_cx(state, n, g[1], g[2])
+        else:
+            raise ValueError(g[0])
     return state
+
+
+def run_gates_quest(qe, qureg, gates) -> None:
+    """The gate list through a QuEST C API binding (QLGate::PerformBasicOperation,
+    QLGate.cpp:652-703, maps each basic operation to one QuEST call)."""
+    L = qe.lib
+    for g in gates:
+        if g[0] == "ry":
+            L.rotateY(qureg, g[1], g[2])
+        elif g[0] == "rz":
+            L.rotateZ(qureg, g[1], g[2])
+        elif g[0] == "h":
+            L.hadamard(qureg, g[1])
+        elif g[0] == "cx":
+            L.controlledNot(qureg, g[1], g[2])
+        else:
+            raise ValueError(g[0])
+
+
+def ae_circuit_dense(v) -> np.ndarray:
+    """ExchangeToYGate + FRy applied gate by gate to |0...0> of q qubits."""
+    q = len(v).bit_length() - 1
+    return run_gates_dense(ae_gates(v, list(range(q))), q).real
 
 
 def ae_state_tree(v) -> np.ndarray:
@@ -146,27 +213,20 @@ def ae_state_tree(v) -> np.ndarray:
 
 def swap_test_circuit_dense(v1, v2) -> float:
     """CreateSwapTestReal's gate list on 2q+1 qubits, P(ancilla = 0)."""
-    D = len(v1)
-    q = D.bit_length() - 1
+    q = len(v1).bit_length() - 1
     n = 2 * q + 1
-    a = ae_circuit_dense(v1)
-    b = ae_circuit_dense(v2)
-    # after H(0) and the two encodings: |+> (x) |a> on 1..q (x) |b> on q+1..2q
-    state = np.zeros(1 << n)
-    for x in range(D):
-        for y in range(D):
-            base = (x << 1) | (y << (q + 1))
-            state[base] = a[x] * b[y] / np.sqrt(2)
-            state[base | 1] = a[x] * b[y] / np.sqrt(2)
-    for i in range(q):  # CSWAP(0, i+1, i+1+q) = CX(t2->t1) CCX(0, t1 -> t2) CX(t2->t1)
-        t1, t2 = i + 1, i + 1 + q
-        state = _apply_cx(state, n, t2, t1)
-        state = _apply_ccx(state, n, 0, t1, t2)
-        state = _apply_cx(state, n, t2, t1)
-    h = np.array([[1, 1], [1, -1]]) / np.sqrt(2)
-    state = _apply_1q(state, n, 0, h)
+    state = run_gates_dense(swap_test_gates(v1, v2), n)
     idx = np.arange(state.size)
     return float(np.sum(np.abs(state[(idx & 1) == 0]) ** 2))
+
+
+def ccx_dense(n, c0, c1, t) -> np.ndarray:
+    """Exact Toffoli as a permutation of the basis (for pinning the decomposition)."""
+    idx = np.arange(1 << n)
+    sel = (((idx >> c0) & 1) == 1) & (((idx >> c1) & 1) == 1)
+    src = idx.copy()
+    src[sel] = idx[sel] ^ (1 << t)
+    return src
 
 
 # ---- k-means in FP64 (closed forms) ----------------------------------------------

@@ -539,6 +539,18 @@ void qmxSwapPlanCentroids(qmxSwapPlan *plan, double *out) {
     deviceToHost(out, s->cent, (size_t)s->C * (1ull << s->maxQ) * sizeof(double));
 }
 
+// replace the centroids (numCentroids x 2^maxQubits doubles) and re-encode
+// them; the point registers, tiles, autotuned shapes and a captured graph stay
+void qmxSwapPlanSetCentroids(qmxSwapPlan *plan, const double *centroids) {
+    using namespace qmx;
+    SwapPlanImpl *s = &plan->impl;
+    deviceSync();
+    hostToDevice(s->cent, centroids, (size_t)s->C * (1ull << s->maxQ) * sizeof(double));
+    QMX_DISPATCH(s->prec, T, { encodeCentroids<T>(s); });
+    checkLaunch("qmxSwapPlanSetCentroids");
+    deviceSync();
+}
+
 // encoded state of point p (its 2^q amplitudes), for checking the encoder
 void qmxSwapPlanPointState(qmxSwapPlan *plan, int p, double *out) {
     using namespace qmx;

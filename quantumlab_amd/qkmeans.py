@@ -86,6 +86,12 @@ class SwapPlan:
         self.lib.qmxSwapPlanCentroids(self.plan, out.ctypes.data_as(C.POINTER(C.c_double)))
         return out.reshape(self.C, -1)
 
+    def set_centroids(self, centroids: np.ndarray):
+        cents = np.ascontiguousarray(centroids, dtype=np.float64)
+        if cents.shape != (self.C, 1 << self.data.max_q):
+            raise ValueError(f"centroids must be {self.C} x {1 << self.data.max_q}")
+        self.lib.qmxSwapPlanSetCentroids(self.plan, cents.ctypes.data_as(C.POINTER(C.c_double)))
+
     def point_state(self, p: int) -> np.ndarray:
         out = np.empty(1 << int(self.data.qubits[p]))
         self.lib.qmxSwapPlanPointState(self.plan, p, out.ctypes.data_as(C.POINTER(C.c_double)))

@@ -266,6 +266,7 @@ _EXT_SIGS = {
     "qmxSwapPlanAssignments": (None, [C.c_void_p, C.POINTER(C.c_int)]),
     "qmxSwapPlanCentroids": (None, [C.c_void_p, C.POINTER(C.c_double)]),
     "qmxSwapPlanPointState": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "qmxSwapPlanSetCentroids": (None, [C.c_void_p, C.POINTER(C.c_double)]),
     "qmxMtInitByArray": (None, [C.POINTER(C.c_ulong), C.c_int]),
     "qmxMtNextU32": (C.c_uint, []),
     "qmxMtNextReal1": (C.c_double, []),

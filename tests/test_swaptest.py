@@ -28,6 +28,20 @@ def test_encoder_handles_zero_blocks():
     assert np.max(np.abs(O.ae_state_tree(v) - v / np.linalg.norm(v))) < 1e-13
 
 
+def test_toffoli_decomposition_is_a_toffoli():
+    """The H/CX/RZ(pi/4) list equals the basis permutation of CCX up to one
+    global phase, for every control/target placement on 4 qubits."""
+    rng = np.random.default_rng(0)
+    for c0, c1, t in [(0, 1, 2), (0, 2, 3), (3, 1, 0), (2, 0, 1)]:
+        psi = rng.standard_normal(16) + 1j * rng.standard_normal(16)
+        psi /= np.linalg.norm(psi)
+        state = O.run_gates_dense(O.toffoli_gates(c0, c1, t), 4, psi.copy())
+        want = psi[O.ccx_dense(4, c0, c1, t)]
+        phase = np.vdot(want, state)
+        assert abs(abs(phase) - 1) < 1e-13
+        assert np.max(np.abs(state - phase * want)) < 1e-13
+
+
 @pytest.mark.parametrize("q", [1, 2, 3])
 def test_gate_level_swap_test_matches_closed_form(q):
     rng = np.random.default_rng(10 + q)
@@ -36,6 +50,24 @@ def test_gate_level_swap_test_matches_closed_form(q):
     ov = (a @ b) / (np.linalg.norm(a) * np.linalg.norm(b))
     assert abs(p0 - 0.5 * (1 + ov ** 2)) < 1e-13
     assert abs(O.swap_test_circuit_dense(a, a) - 1.0) < 1e-13
+
+
+@pytest.mark.parametrize("lib", ["reflib", "cpuref"])
+def test_swap_test_gate_list_through_quest_cpu(lib, request):
+    """The same gate list through QuEST's C API on the CPU (QuEST 3.4's own
+    backend, and the C restatement) gives the dense state and the closed-form
+    P(ancilla = 0)."""
+    qe = request.getfixturevalue(lib)
+    rng = np.random.default_rng(21)
+    for q in (1, 2, 3):
+        a, b = rng.standard_normal(1 << q), rng.standard_normal(1 << q)
+        gates = O.swap_test_gates(a, b)
+        reg = qe.qureg(2 * q + 1)
+        O.run_gates_quest(qe, reg, gates)
+        ov = (a @ b) / (np.linalg.norm(a) * np.linalg.norm(b))
+        assert abs(qe.lib.calcProbOfOutcome(reg, 0, 0) - 0.5 * (1 + ov ** 2)) < 1e-12
+        assert np.max(np.abs(qe.state(reg) - O.run_gates_dense(gates, 2 * q + 1))) < 1e-12
+        qe.destroy(reg)
 
 
 def test_kmeans_oracle_recovers_clusters():
@@ -116,6 +148,24 @@ def test_graph_replays_match_eager_iterations(gpu):
         assert np.array_equal(eager.centroids(), graph.centroids()), it
     eager.close()
     graph.close()
+
+
+@pytest.mark.gpu
+def test_set_centroids_restarts_the_iteration(gpu):
+    """Resetting the centroids after graph replays gives the same iterations as
+    a fresh plan (the captured graph keeps reading the plan's buffers)."""
+    data = _small(seed=17, P=512, C=4)
+    fresh = K.SwapPlan(gpu, data, QMX_PREC_FP32)
+    fresh.run(3, graph=False)
+    plan = K.SwapPlan(gpu, data, QMX_PREC_FP32)
+    plan.run(2, graph=True)
+    plan.set_centroids(data.init)
+    plan.run(3, graph=True)
+    gpu.qmxSync()
+    assert np.array_equal(plan.p0(), fresh.p0())
+    assert np.array_equal(plan.centroids(), fresh.centroids())
+    fresh.close()
+    plan.close()
 
 
 @pytest.mark.gpu
