@@ -26,6 +26,11 @@ update per iteration, FP32 registers and FP64 sums, one HIP graph per
 iteration) in swap-tests/s, with its distance error against exact FP64 inner
 products, assignment agreement with the FP64 run, and its own CPU baseline:
 the reference's gate-by-gate swap-test circuits on QuEST's CPU backend.
+
+`density`: the noisy brick circuit (60 layers, dephasing + depolarising +
+damping after every gate on its qubits) on a 14-qubit density matrix in FP64
+and FP32, with purity, per-qubit P(0), fidelity with the noiseless run, the
+FP32 form's errors against FP64, and QuEST's CPU backend at 13 qubits.
 """
 from __future__ import annotations
 
@@ -136,6 +141,68 @@ def cpu_baseline(n_cpu, seed):
                       f"through QuEST's C API; {'QuEST 3.4 CPU backend compiled from the reference sources' if kind == 'reference' else 'C restatement of it'}, "
                       f"OpenMP {threads} threads; {dt:.2f} s",
             "gates_per_s": round(len(gates) / dt, 2)}
+
+
+def run_density(qe, n, layers, seed):
+    """Noisy brick circuit on an n-qubit density matrix in FP64 and in FP32
+    (FP64 reductions): wall time, algorithmic GB/s, purity, per-qubit P(0),
+    fidelity with the noiseless pure-state run, and the FP32 form's errors
+    against the FP64 form."""
+    from quantumlab_amd import workloads as W
+    from quantumlab_amd.quest import QMX_PREC_FP32, QMX_PREC_FP64
+    gates = W.noisy_brick_circuit(n, layers, seed)
+    pure = qe.qureg(n)
+    W.apply_gates(qe, pure, W.unitary_part(gates))
+    res = {"workload": f"{n}-qubit density matrix (4^{n} entries, {(1 << 2 * n) * 16 / 2**30:.0f} GiB FP64) from |0><0|, "
+                       f"{layers} brick layers, mixDephasing + mixDepolarising + mixDamping after every gate on "
+                       f"each qubit it touches",
+           "operations": len(gates), "channels": sum(1 for g in gates if g[0] in W.CHANNELS)}
+    outs = {}
+    for name, prec, B in (("fp64", QMX_PREC_FP64, 16), ("fp32", QMX_PREC_FP32, 8)):
+        rho = qe.qureg(n, density=True, precision=prec)
+        qe.qmxSync()
+        t0 = time.perf_counter()
+        W.apply_gates(qe, rho, gates)
+        qe.qmxSync()
+        wall = time.perf_counter() - t0
+        probs = [qe.calcProbOfOutcome(rho, k, 0) for k in range(n)]
+        outs[name] = {"seconds": round(wall, 4), "ms_per_layer": round(wall / layers * 1e3, 3),
+                      "gb_per_s": round(W.density_bytes(gates, n, B) / wall / 1e9, 1),
+                      "purity": qe.calcPurity(rho), "fidelity_vs_noiseless": qe.calcFidelity(rho, pure),
+                      "trace": qe.calcTotalProb(rho), "prob0": probs}
+        qe.destroy(rho)
+        log(f"density {n}q {name}: {wall:.2f} s, {outs[name]['gb_per_s']} GB/s, purity {outs[name]['purity']:.6f}, "
+            f"F {outs[name]['fidelity_vs_noiseless']:.6f}")
+    qe.destroy(pure)
+    a, b = outs["fp64"], outs["fp32"]
+    res.update(outs)
+    res["fp32_vs_fp64"] = {"purity_error": abs(a["purity"] - b["purity"]),
+                           "fidelity_error": abs(a["fidelity_vs_noiseless"] - b["fidelity_vs_noiseless"]),
+                           "max_prob_error": max(abs(x - y) for x, y in zip(a["prob0"], b["prob0"])),
+                           "speedup": round(a["seconds"] / b["seconds"], 3)}
+    return res
+
+
+def cpu_density_baseline(n_cpu, layers, seed):
+    """QuEST's CPU backend on the noisy brick circuit at n_cpu qubits (density)."""
+    from quantumlab_amd import workloads as W
+    from quantumlab_amd.quest import QuEST
+    ref = os.path.join(ROOT, "oracle", "_ref", "libquest_ref.so")
+    port = os.path.join(ROOT, "oracle", "build", "libcpuref.so")
+    kind, path = ("reference", ref) if os.path.exists(ref) else ("port", port)
+    lib = QuEST(path)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    gates = W.noisy_brick_circuit(n_cpu, layers, seed)
+    rho = lib.qureg(n_cpu, density=True)
+    t0 = time.perf_counter()
+    W.apply_gates(lib, rho, gates)
+    dt = time.perf_counter() - t0
+    lib.destroy(rho)
+    return {"value": round(W.density_bytes(gates, n_cpu, 16) / dt / 1e9, 3), "unit": "GB/s", "cores": threads,
+            "kind": kind, "ms_per_layer": round(dt / layers * 1e3, 2),
+            "sample": f"{n_cpu}-qubit FP64 density matrix, {layers} noisy brick layers ({len(gates)} calls, same "
+                      f"generator); {'QuEST 3.4 CPU backend compiled from the reference sources' if kind == 'reference' else 'C restatement of it'}, "
+                      f"OpenMP {threads} threads; {dt:.2f} s"}
 
 
 def exact_p0(data, cents):
@@ -268,6 +335,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--no-swap", action="store_true")
+    ap.add_argument("--no-density", action="store_true")
+    ap.add_argument("--density-qubits", type=int, default=14)
+    ap.add_argument("--density-layers", type=int, default=60)
+    ap.add_argument("--cpu-density-qubits", type=int, default=13)
     ap.add_argument("--swap-points", type=int, default=16384)
     ap.add_argument("--swap-centroids", type=int, default=16)
     ap.add_argument("--swap-iterations", type=int, default=10)
@@ -357,6 +428,9 @@ def main():
         qe.destroy(q32)
     qe.destroy(q64)
 
+    if not args.no_density:
+        result["density"] = run_density(qe, args.density_qubits, args.density_layers, args.seed)
+
     swap_data = None
     if not args.no_swap:
         result["swap_tests"], swap_data = run_swap_tests(qe, args.swap_points, args.swap_centroids,
@@ -370,6 +444,9 @@ def main():
         log(f"cpu baseline at {args.cpu_qubits} qubits ...")
         result["cpu_baseline"] = cpu_baseline(args.cpu_qubits, args.seed)
         log(f"cpu baseline: {result['cpu_baseline']['value']} GB/s")
+        if "density" in result:
+            result["density"]["cpu_baseline"] = cpu_density_baseline(args.cpu_density_qubits, 2, args.seed)
+            log(f"cpu density baseline: {result['density']['cpu_baseline']['value']} GB/s")
         if swap_data is not None:
             result["swap_tests"]["cpu_baseline"] = cpu_swap_baseline(swap_data)
             log(f"cpu swap-test baseline: {result['swap_tests']['cpu_baseline']['value']} swap-tests/s")

@@ -15,7 +15,14 @@ A gate list is a list of tuples:
     ("rz", target, theta)                rotateZ
     ("mcu", controls, target, u2x2)      multiControlledUnitary
     ("mqu", targets, u)                  multiQubitUnitary (targets[0] = LSB)
+    ("deph", target, p)                  mixDephasing       (density matrices)
+    ("depol", target, p)                 mixDepolarising
+    ("damp", target, gamma)              mixDamping
 The same list feeds the MI355X library, the oracle and the reference build.
+
+Noisy brick circuit (density-matrix workload): the brick circuit with
+mixDephasing(p ~ U(0, 0.02)), mixDepolarising(p ~ U(0, 0.01)) and
+mixDamping(gamma ~ U(0, 0.02)) after every gate on each qubit it touches.
 """
 from __future__ import annotations
 
@@ -78,6 +85,26 @@ def brick_circuit(n: int, layers: int, seed: int, extra_every: int = 0, first_la
     return gates
 
 
+CHANNELS = ("deph", "depol", "damp")
+
+
+def noisy_brick_circuit(n: int, layers: int, seed: int) -> List[tuple]:
+    rng = np.random.default_rng([seed, 99])
+    out: List[tuple] = []
+    for g in brick_circuit(n, layers, seed):
+        out.append(g)
+        touched = [g[1]] if g[0] in ("cu", "rz") else [g[1], g[2]]
+        for t in touched:
+            out.append(("deph", t, float(rng.uniform(0, 0.02))))
+            out.append(("depol", t, float(rng.uniform(0, 0.01))))
+            out.append(("damp", t, float(rng.uniform(0, 0.02))))
+    return out
+
+
+def unitary_part(gates: Sequence[tuple]) -> List[tuple]:
+    return [g for g in gates if g[0] not in CHANNELS]
+
+
 def apply_gates(qe: QuEST, q, gates: Sequence[tuple]):
     """Apply a gate list through the QuEST C API of any loaded library."""
     for g in gates:
@@ -92,6 +119,12 @@ def apply_gates(qe: QuEST, q, gates: Sequence[tuple]):
             qe.multiControlledUnitary(q, g[1], g[2], g[3])
         elif kind == "mqu":
             qe.multiQubitUnitary(q, g[1], g[2])
+        elif kind == "deph":
+            qe.call("mixDephasing", q, g[1], g[2])
+        elif kind == "depol":
+            qe.call("mixDepolarising", q, g[1], g[2])
+        elif kind == "damp":
+            qe.call("mixDamping", q, g[1], g[2])
         else:
             raise ValueError(f"unknown gate {kind}")
 
@@ -136,4 +169,22 @@ def gate_bytes(gates: Sequence[tuple], n: int, amp_bytes: int) -> float:
             tot += N * amp_bytes
         elif g[0] == "mcu":
             tot += 2 * N * amp_bytes / (1 << len(g[1]))
+    return tot
+
+
+def density_bytes(gates: Sequence[tuple], n: int, amp_bytes: int) -> float:
+    """Algorithmic HBM bytes of a gate list on an n-qubit density matrix
+    (N_rho = 4^n entries): a unitary is the same gate on the 2n-qubit vector
+    twice (rows, then conjugated on columns); mixDephasing reads and rewrites
+    the half of the entries whose row and column bits differ (N_rho*B);
+    mixDepolarising and mixDamping rewrite every entry (2*N_rho*B)."""
+    Nr = float(1 << (2 * n))
+    tot = 0.0
+    for g in gates:
+        if g[0] == "deph":
+            tot += Nr * amp_bytes
+        elif g[0] in ("depol", "damp"):
+            tot += 2 * Nr * amp_bytes
+        else:
+            tot += 2 * gate_bytes([g], 2 * n, amp_bytes)
     return tot

@@ -93,13 +93,6 @@ def channel_circuit(qe: QuEST, rho, pure, n: int, layers: int, seed: int):
     """Brick layers on a density matrix with dephasing, depolarising and damping
     after every gate on each qubit it touches; the same gates, noiselessly, on
     `pure` (a state vector)."""
-    rng = np.random.default_rng([seed, 99])
-    gates = W.brick_circuit(n, layers, seed)
-    for g in gates:
-        W.apply_gates(qe, rho, [g])
-        W.apply_gates(qe, pure, [g])
-        touched = [g[1]] if g[0] in ("cu", "rz") else [g[1], g[2]]
-        for t in touched:
-            qe.call("mixDephasing", rho, t, float(rng.uniform(0, 0.02)))
-            qe.call("mixDepolarising", rho, t, float(rng.uniform(0, 0.01)))
-            qe.call("mixDamping", rho, t, float(rng.uniform(0, 0.02)))
+    gates = W.noisy_brick_circuit(n, layers, seed)
+    W.apply_gates(qe, rho, gates)
+    W.apply_gates(qe, pure, W.unitary_part(gates))
