@@ -106,6 +106,76 @@ void qmxSwapPlanSetCentroids(qmxSwapPlan *plan, const double *centroids);
 /* the encoded register state of point p (2^q amplitudes) */
 void qmxSwapPlanPointState(qmxSwapPlan *plan, int p, double *out);
 
+/* ---- Trotter evolution of many small registers (planner labels) ------------
+ * Register j (hamils[j].numQubits <= 12 qubits, FP64, held in LDS) starts in
+ * the product state prod_q rotateZ(phi_q) rotateY(theta_q)|0> (theta, phi packed
+ * per register: hamils[0].numQubits values, then hamils[1]'s, ...) and is
+ * evolved exactly as applyTrotterCircuit(reg, hamils[j], times[j], orders[j],
+ * reps) would (QuEST.h:5450, QuEST_common.c:762-844). Return 1, or 0 on invalid
+ * input (message on stderr). */
+/* final states, packed 2^n amplitudes per register */
+int qmxTrotterBatchEvolve(int numJobs, const PauliHamil *hamils, const qreal *times, const qreal *theta,
+                          const qreal *phi, const int *orders, const int *reps, qreal *outReal, qreal *outImag);
+/* ladders: for r = 2^k, k = 0..maxLog2, infidelity[j (maxLog2+1) + k] =
+ * 1 - |<psi_r|psi_4r>|^2 until the first value <= eps (its k in passIndex[j],
+ * -1 if none); entries not reached are NaN */
+int qmxTrotterBatchLadder(int numJobs, const PauliHamil *hamils, const qreal *times, const qreal *theta,
+                          const qreal *phi, const int *orders, int maxLog2, qreal eps, qreal *infidelity,
+                          int *passIndex);
+
+/* ---- learned Trotter planner (no counterpart in the reference) -------------
+ * Picks the order (2 or 4) and the repetitions for applyTrotterCircuit that
+ * reach a fidelity target, from 16 per-qubit features of the Pauli sum:
+ *   0 t, 1 log2 t, 2 terms/n, 3 lambda/n (lambda = sum |c_j|), 4 log2(lambda/n),
+ *   5 max |c_j|, 6 ||c||_2/sqrt(n), 7 log2(C1/n), 8 log2(C2/n),
+ *   9 log2(lambda t/n), 10 log2(C1 t^2/n), 11 log2(C2 t^3/n),
+ *   12 anticommuting partners per term, 13 mean Pauli weight, 14 max weight,
+ *   15 share of lambda on terms of weight >= 3,
+ * with C1 = sum_{j<k} ||[c_j P_j, c_k P_k]|| and
+ * C2 = sum_{j<k} sum_l ||[c_l P_l, [c_j P_j, c_k P_k]]|| (identity strings
+ * dropped; logs floored at -60). The network (16 -> 64 -> 64 -> 2, ReLU,
+ * bfloat16-rounded weights, FP32 arithmetic) predicts the size-free
+ * y_o = log2 r*_o - beta_o log2(n) for o = 2, 4; the planner takes
+ * r_o = ceil(2^(y_o + beta_o log2(n) + margin)) and the order with fewer Pauli
+ * exponentials (stages x r). beta_o (the size exponents, default 1/(2o)) and
+ * the margin are set by training. */
+typedef struct qmxPlanner qmxPlanner;
+int qmxTrotterFeatures(PauliHamil hamil, qreal time, float *out16);
+/* Pauli-exponential passes per repetition: 1 (order 1), 2, 10, 50, ... */
+int qmxTrotterStages(int order);
+qmxPlanner *qmxPlannerCreate(unsigned seed);  /* He-uniform weights, zero biases */
+void qmxPlannerDestroy(qmxPlanner *p);
+int qmxPlannerNumParams(void);                 /* 5378 */
+/* flat FP32 parameters: W1 (16x64 row-major), b1, W2 (64x64), b2, W3 (64x2), b3 */
+void qmxPlannerGetParams(const qmxPlanner *p, float *out);
+void qmxPlannerSetParams(qmxPlanner *p, const float *params);  /* also resets Adam */
+/* inputs are standardised as (x - mean) / scale before the first layer */
+void qmxPlannerSetNormalisation(qmxPlanner *p, const float *mean16, const float *scale16);
+void qmxPlannerGetNormalisation(const qmxPlanner *p, float *mean16, float *scale16);
+void qmxPlannerSetMargin(qmxPlanner *p, float log2Margin);
+float qmxPlannerGetMargin(const qmxPlanner *p);
+void qmxPlannerSetSizeExponents(qmxPlanner *p, const float *beta2);
+void qmxPlannerGetSizeExponents(const qmxPlanner *p, float *beta2);
+/* Y (N x 2) from X (N x 16) */
+void qmxPlannerPredict(qmxPlanner *p, const float *X, int N, float *Y);
+/* masked mean squared error sum(M (Y - T)^2) / max(1, sum M) and its gradient
+ * (5378 floats) with respect to the rounded parameters; no update */
+double qmxPlannerGradient(qmxPlanner *p, const float *X, const float *T, const float *M, int N, float *grad);
+/* Adam (0.9, 0.999, 1e-8) over shuffled minibatches; returns the last epoch's
+ * mean minibatch loss; lossPerEpoch (epochs floats) may be NULL */
+double qmxPlannerTrain(qmxPlanner *p, const float *X, const float *T, const float *M, int N, int epochs, int batch,
+                       float lr, unsigned seed, float *lossPerEpoch);
+int qmxPlannerChoose(qmxPlanner *p, PauliHamil hamil, qreal time, int *order, int *reps);
+/* applyTrotterCircuit at the planner's choice */
+int qmxApplyTrotterLearned(Qureg qureg, PauliHamil hamil, qreal time, qmxPlanner *p, int *order, int *reps);
+/* plain step doubling at a fixed order (the planner's fallback and judge):
+ * r = 1, 2, 4, ... from the initial state until 1 - |<psi_{r/2}|psi_r>|^2 <= eps;
+ * qureg ends as psi_r; returns 1 if eps was met within maxReps */
+int qmxApplyTrotterStepDoubling(Qureg qureg, PauliHamil hamil, qreal time, int order, qreal eps, int maxReps,
+                                int *reps);
+int qmxPlannerSave(const qmxPlanner *p, const char *path);
+qmxPlanner *qmxPlannerLoad(const char *path);
+
 #ifdef __cplusplus
 }
 #endif

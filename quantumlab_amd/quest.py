@@ -267,6 +267,37 @@ _EXT_SIGS = {
     "qmxSwapPlanCentroids": (None, [C.c_void_p, C.POINTER(C.c_double)]),
     "qmxSwapPlanPointState": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "qmxSwapPlanSetCentroids": (None, [C.c_void_p, C.POINTER(C.c_double)]),
+    "qmxTrotterBatchEvolve": (C.c_int, [C.c_int, C.POINTER(PauliHamil), C.POINTER(qreal), C.POINTER(qreal),
+                                        C.POINTER(qreal), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(qreal),
+                                        C.POINTER(qreal)]),
+    "qmxTrotterBatchLadder": (C.c_int, [C.c_int, C.POINTER(PauliHamil), C.POINTER(qreal), C.POINTER(qreal),
+                                        C.POINTER(qreal), C.POINTER(C.c_int), C.c_int, qreal, C.POINTER(qreal),
+                                        C.POINTER(C.c_int)]),
+    "qmxTrotterFeatures": (C.c_int, [PauliHamil, qreal, C.POINTER(C.c_float)]),
+    "qmxTrotterStages": (C.c_int, [C.c_int]),
+    "qmxPlannerCreate": (C.c_void_p, [C.c_uint]),
+    "qmxPlannerDestroy": (None, [C.c_void_p]),
+    "qmxPlannerNumParams": (C.c_int, []),
+    "qmxPlannerGetParams": (None, [C.c_void_p, C.POINTER(C.c_float)]),
+    "qmxPlannerSetParams": (None, [C.c_void_p, C.POINTER(C.c_float)]),
+    "qmxPlannerSetNormalisation": (None, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "qmxPlannerGetNormalisation": (None, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "qmxPlannerSetMargin": (None, [C.c_void_p, C.c_float]),
+    "qmxPlannerGetMargin": (C.c_float, [C.c_void_p]),
+    "qmxPlannerSetSizeExponents": (None, [C.c_void_p, C.POINTER(C.c_float)]),
+    "qmxPlannerGetSizeExponents": (None, [C.c_void_p, C.POINTER(C.c_float)]),
+    "qmxPlannerPredict": (None, [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
+    "qmxPlannerGradient": (C.c_double, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                        C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
+    "qmxPlannerTrain": (C.c_double, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                     C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint, C.POINTER(C.c_float)]),
+    "qmxPlannerChoose": (C.c_int, [C.c_void_p, PauliHamil, qreal, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "qmxApplyTrotterLearned": (C.c_int, [Qureg, PauliHamil, qreal, C.c_void_p, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]),
+    "qmxApplyTrotterStepDoubling": (C.c_int, [Qureg, PauliHamil, qreal, C.c_int, qreal, C.c_int,
+                                              C.POINTER(C.c_int)]),
+    "qmxPlannerSave": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "qmxPlannerLoad": (C.c_void_p, [C.c_char_p]),
     "qmxMtInitByArray": (None, [C.POINTER(C.c_ulong), C.c_int]),
     "qmxMtNextU32": (C.c_uint, []),
     "qmxMtNextReal1": (C.c_double, []),
