@@ -116,6 +116,9 @@ void qmxSwapPlanPointState(qmxSwapPlan *plan, int p, double *out);
 /* final states, packed 2^n amplitudes per register */
 int qmxTrotterBatchEvolve(int numJobs, const PauliHamil *hamils, const qreal *times, const qreal *theta,
                           const qreal *phi, const int *orders, const int *reps, qreal *outReal, qreal *outImag);
+/* infidelity[j] = 1 - |<psi_r|psi_4r>|^2 with r = reps[j] (the label judge) */
+int qmxTrotterBatchInfidelity(int numJobs, const PauliHamil *hamils, const qreal *times, const qreal *theta,
+                              const qreal *phi, const int *orders, const int *reps, qreal *infidelity);
 /* ladders: for r = 2^k, k = 0..maxLog2, infidelity[j (maxLog2+1) + k] =
  * 1 - |<psi_r|psi_4r>|^2 until the first value <= eps (its k in passIndex[j],
  * -1 if none); entries not reached are NaN */
@@ -134,11 +137,12 @@ int qmxTrotterBatchLadder(int numJobs, const PauliHamil *hamils, const qreal *ti
  * with C1 = sum_{j<k} ||[c_j P_j, c_k P_k]|| and
  * C2 = sum_{j<k} sum_l ||[c_l P_l, [c_j P_j, c_k P_k]]|| (identity strings
  * dropped; logs floored at -60). The network (16 -> 64 -> 64 -> 2, ReLU,
- * bfloat16-rounded weights, FP32 arithmetic) predicts the size-free
- * y_o = log2 r*_o - beta_o log2(n) for o = 2, 4; the planner takes
- * r_o = ceil(2^(y_o + beta_o log2(n) + margin)) and the order with fewer Pauli
- * exponentials (stages x r). beta_o (the size exponents, default 1/(2o)) and
- * the margin are set by training. */
+ * bfloat16-rounded weights, FP32 arithmetic) predicts the deviation
+ * y_o = log2 r*_o - curve_o(n) for o = 2, 4 from a size curve (mean log2 r*_o
+ * per register size, piecewise linear in log2 n, extended past its ends with
+ * the end slopes; default log2(n)/(2o)); the planner takes
+ * r_o = ceil(2^(y_o + curve_o(n) + margin)) and the order with fewer Pauli
+ * exponentials (stages x r). Training sets the curve and the margin. */
 typedef struct qmxPlanner qmxPlanner;
 int qmxTrotterFeatures(PauliHamil hamil, qreal time, float *out16);
 /* Pauli-exponential passes per repetition: 1 (order 1), 2, 10, 50, ... */
@@ -154,8 +158,11 @@ void qmxPlannerSetNormalisation(qmxPlanner *p, const float *mean16, const float 
 void qmxPlannerGetNormalisation(const qmxPlanner *p, float *mean16, float *scale16);
 void qmxPlannerSetMargin(qmxPlanner *p, float log2Margin);
 float qmxPlannerGetMargin(const qmxPlanner *p);
-void qmxPlannerSetSizeExponents(qmxPlanner *p, const float *beta2);
-void qmxPlannerGetSizeExponents(const qmxPlanner *p, float *beta2);
+/* knots: log2 n ascending, log2reps knots x 2 (orders 2, 4); returns 1, or 0 if invalid */
+int qmxPlannerSetSizeCurve(qmxPlanner *p, int knots, const float *log2n, const float *log2reps);
+/* returns the knot count; either array may be NULL */
+int qmxPlannerGetSizeCurve(const qmxPlanner *p, float *log2n, float *log2reps);
+double qmxPlannerSizeCurve(const qmxPlanner *p, int order, int numQubits);
 /* Y (N x 2) from X (N x 16) */
 void qmxPlannerPredict(qmxPlanner *p, const float *X, int N, float *Y);
 /* masked mean squared error sum(M (Y - T)^2) / max(1, sum M) and its gradient
@@ -169,7 +176,8 @@ int qmxPlannerChoose(qmxPlanner *p, PauliHamil hamil, qreal time, int *order, in
 /* applyTrotterCircuit at the planner's choice */
 int qmxApplyTrotterLearned(Qureg qureg, PauliHamil hamil, qreal time, qmxPlanner *p, int *order, int *reps);
 /* plain step doubling at a fixed order (the planner's fallback and judge):
- * r = 1, 2, 4, ... from the initial state until 1 - |<psi_{r/2}|psi_r>|^2 <= eps;
+ * r = 1, 2, 4, ... from the initial state until the normalised infidelity
+ * 1 - |<psi_{r/2}|psi_r>|^2 / (|psi_{r/2}|^2 |psi_r|^2) <= eps;
  * qureg ends as psi_r; returns 1 if eps was met within maxReps */
 int qmxApplyTrotterStepDoubling(Qureg qureg, PauliHamil hamil, qreal time, int order, qreal eps, int maxReps,
                                 int *reps);

@@ -125,13 +125,24 @@ def features(codes, coeffs, time: float) -> np.ndarray:
     return np.asarray(f, dtype=np.float64)
 
 
-def choose(y, n: int, margin: float, beta=(0.25, 0.125)):
+def size_curve(kx, ky, n: int, o: int) -> float:
+    """Piecewise-linear interpolation in log2 n through the knots, the end
+    segments extended (restated from the planner's definition)."""
+    x = np.log2(n)
+    k = 0
+    while k < len(kx) - 2 and x > kx[k + 1]:
+        k += 1
+    return ky[k][o] + (ky[k + 1][o] - ky[k][o]) * (x - kx[k]) / (kx[k + 1] - kx[k])
+
+
+def choose(y, n: int, margin: float, kx=(0.0, 1.0), ky=((0.0, 0.0), (0.25, 0.125))):
     """The planner's decision from its two outputs (orders 2 and 4): repetitions
-    ceil(2^(y_o + beta_o log2(n) + margin)) and the order with fewer Pauli
+    ceil(2^(y_o + curve_o(n) + margin)) and the order with fewer Pauli
     exponentials (2 r_2 versus 10 r_4; ties go to order 2)."""
     best = None
-    for o, yo, stages, b in ((2, y[0], 2, beta[0]), (4, y[1], 10, beta[1])):
-        r = max(1, int(np.ceil(2.0 ** min(30.0, yo + b * np.log2(n) + margin) - 1e-9)))
+    for c, (o, stages) in enumerate(((2, 2), (4, 10))):
+        l2 = float(y[c]) + size_curve(kx, ky, n, c) + margin
+        r = max(1, int(np.ceil(2.0 ** min(30.0, l2) - 1e-9)))
         if best is None or stages * r < best[2]:
             best = (o, r, stages * r)
     return best[0], best[1]

@@ -96,6 +96,15 @@ void fused5Apply(QuregDev *q, const void *prepared);
 // exp(-i angle/2 * P), P = X^{xMask} Z^{zMask} with Y = iXZ folded in (single sweep)
 void applyPauliRotation(QuregDev *q, uint64_t ctrlMask, uint64_t xMask, uint64_t zMask,
                         int numY, double angle);
+// an ordered list of rotations exp(-i angle/2 P) applied `reps` times, runs of
+// rotations whose X/Y supports fit one LDS tile fused into one sweep (qmx_pauli.hip)
+struct PauliExp {
+    uint64_t x, z;
+    int ny;
+    double angle;
+};
+void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps);
+int pauliSequenceBlocks(int numQubits, int prec, const PauliExp *seq, int count);
 
 // ---- reductions (qmx_reduce.hip); all accumulate in FP64 -------------------
 double probOfZero(const QuregDev *q, int qubit);                   // state vector

@@ -5,13 +5,13 @@
 //
 // Target the network learns. For order o the infidelity of o's formula at r
 // repetitions behaves as 1 - F ~ A_o r^(-2o), and A_o of a local Hamiltonian
-// grows with the register size n roughly as a power n^k (k = 1 for a sum of
-// independent local errors), so log2 r*_o = log2(A_o / eps) / (2o) grows as
-// beta_o log2(n) with beta_o ~ k / (2o). The features are per-qubit
-// (intensive) quantities, the network predicts the size-free part
-// y_o = log2 r*_o - beta_o log2(n), and the planner adds beta_o log2(n) back;
-// beta_o is fitted across the training sizes (default 1/(2o)). That lets
-// labels computed on small registers serve large ones.
+// grows with the register size n (roughly as n for a sum of independent local
+// errors), so the cheapest log2 r*_o = log2(A_o / eps) / (2o) drifts slowly
+// with log2(n). The planner keeps that drift as a size curve — the mean
+// log2 r*_o per register size, piecewise linear in log2(n), measured on the
+// training sizes and at the deployment size — and the network, fed per-qubit
+// (intensive) features, predicts the deviation y_o = log2 r*_o - curve_o(n).
+// That lets labels computed on small registers serve large ones.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -127,7 +127,7 @@ int qmxPlannerChoose(qmxPlanner *p, PauliHamil hamil, qreal time, int *order, in
     const int orders[2] = {2, 4};
     long long bestCost = -1;
     for (int o = 0; o < 2; ++o) {
-        const double l2r = y[o] + p->host.sizeExp[o] * std::log2((double)hamil.numQubits) + p->host.margin;
+        const double l2r = y[o] + qmx::sizeCurve(p->host, o, std::log2((double)hamil.numQubits)) + p->host.margin;
         const double r = std::ceil(std::exp2(std::min(30.0, l2r)) - 1e-9);
         const int ri = (int)std::max(1.0, r);
         const long long cost = (long long)stagesOf(orders[o]) * ri;
@@ -141,7 +141,8 @@ int qmxPlannerChoose(qmxPlanner *p, PauliHamil hamil, qreal time, int *order, in
 }
 
 // plain step doubling: r = 1, 2, 4, ...; each rung evolves a copy of the
-// initial state; stop when 1 - |<psi_{r/2}|psi_r>|^2 <= eps and keep psi_r.
+// initial state; stop when 1 - |<psi_{r/2}|psi_r>|^2 / (|psi_{r/2}|^2 |psi_r|^2)
+// <= eps and keep psi_r.
 // Returns 1 when eps was met (qureg holds psi_r), 0 if maxReps ran out
 // (qureg holds the finest rung).
 int qmxApplyTrotterStepDoubling(Qureg qureg, PauliHamil hamil, qreal time, int order, qreal eps, int maxReps,
@@ -162,8 +163,10 @@ int qmxApplyTrotterStepDoubling(Qureg qureg, PauliHamil hamil, qreal time, int o
         r *= 2;
         cloneQureg(qureg, init);
         applyTrotterCircuit(qureg, hamil, time, order, r);
+        // normalised: FP32 registers drift in norm by ~1e-5 over long circuits
         const Complex ip = calcInnerProduct(prev, qureg);
-        if (1.0 - (ip.real * ip.real + ip.imag * ip.imag) <= eps) {
+        const double f = (ip.real * ip.real + ip.imag * ip.imag) / (calcTotalProb(prev) * calcTotalProb(qureg));
+        if (1.0 - f <= eps) {
             ok = 1;
             break;
         }
@@ -184,16 +187,38 @@ int qmxApplyTrotterLearned(Qureg qureg, PauliHamil hamil, qreal time, qmxPlanner
     return 1;
 }
 
-void qmxPlannerSetSizeExponents(qmxPlanner *p, const float *beta) {
-    for (int o = 0; o < qmx::kOutputs; ++o) p->host.sizeExp[o] = beta[o];
+int qmxPlannerSetSizeCurve(qmxPlanner *p, int knots, const float *log2n, const float *log2reps) {
+    if (knots < 2 || knots > qmx::kMaxCurve) {
+        std::fprintf(stderr, "libquest_mi355x: qmxPlannerSetSizeCurve: 2..%d knots\n", qmx::kMaxCurve);
+        return 0;
+    }
+    for (int k = 1; k < knots; ++k)
+        if (!(log2n[k] > log2n[k - 1])) {
+            std::fprintf(stderr, "libquest_mi355x: qmxPlannerSetSizeCurve: knots must ascend\n");
+            return 0;
+        }
+    p->host.knots = knots;
+    for (int k = 0; k < knots; ++k) {
+        p->host.knotLog2n[k] = log2n[k];
+        for (int o = 0; o < qmx::kOutputs; ++o) p->host.knotLog2r[k][o] = log2reps[k * qmx::kOutputs + o];
+    }
+    return 1;
 }
 
-void qmxPlannerGetSizeExponents(const qmxPlanner *p, float *beta) {
-    for (int o = 0; o < qmx::kOutputs; ++o) beta[o] = p->host.sizeExp[o];
+int qmxPlannerGetSizeCurve(const qmxPlanner *p, float *log2n, float *log2reps) {
+    for (int k = 0; k < p->host.knots; ++k) {
+        if (log2n) log2n[k] = p->host.knotLog2n[k];
+        for (int o = 0; o < qmx::kOutputs && log2reps; ++o) log2reps[k * qmx::kOutputs + o] = p->host.knotLog2r[k][o];
+    }
+    return p->host.knots;
+}
+
+double qmxPlannerSizeCurve(const qmxPlanner *p, int order, int numQubits) {
+    return qmx::sizeCurve(p->host, order == 4 ? 1 : 0, std::log2((double)numQubits));
 }
 
 // text format: a header line, then the parameters, the normalisation, the
-// margin and the two size exponents
+// margin and the size curve (knot count, then log2 n and two log2 r per knot)
 int qmxPlannerSave(const qmxPlanner *p, const char *path) {
     std::vector<float> w(qmx::kNumParams);
     qmxPlannerGetParams(p, w.data());
@@ -203,7 +228,9 @@ int qmxPlannerSave(const qmxPlanner *p, const char *path) {
     for (float v : w) std::fprintf(f, "%.9g\n", v);
     for (int i = 0; i < qmx::kFeatures; ++i) std::fprintf(f, "%.9g %.9g\n", p->host.mean[i], p->host.scale[i]);
     std::fprintf(f, "%.9g\n", p->host.margin);
-    std::fprintf(f, "%.9g %.9g\n", p->host.sizeExp[0], p->host.sizeExp[1]);
+    std::fprintf(f, "%d\n", p->host.knots);
+    for (int k = 0; k < p->host.knots; ++k)
+        std::fprintf(f, "%.9g %.9g %.9g\n", p->host.knotLog2n[k], p->host.knotLog2r[k][0], p->host.knotLog2r[k][1]);
     return std::fclose(f) == 0;
 }
 
@@ -215,10 +242,14 @@ qmxPlanner *qmxPlannerLoad(const char *path) {
     bool ok = std::fscanf(f, "%63s %d %d %d", tag, &ver, &np, &nf) == 4 && std::string(tag) == "qmx-trotter-planner" &&
               ver == 1 && np == qmx::kNumParams && nf == qmx::kFeatures;
     std::vector<float> w(qmx::kNumParams);
-    float mean[qmx::kFeatures], scale[qmx::kFeatures], margin = 0, beta[qmx::kOutputs] = {0, 0};
+    float mean[qmx::kFeatures], scale[qmx::kFeatures], margin = 0;
+    int knots = 0;
+    float kx[qmx::kMaxCurve], ky[qmx::kMaxCurve * qmx::kOutputs];
     for (int i = 0; ok && i < qmx::kNumParams; ++i) ok = std::fscanf(f, "%f", &w[i]) == 1;
     for (int i = 0; ok && i < qmx::kFeatures; ++i) ok = std::fscanf(f, "%f %f", &mean[i], &scale[i]) == 2;
-    ok = ok && std::fscanf(f, "%f", &margin) == 1 && std::fscanf(f, "%f %f", &beta[0], &beta[1]) == 2;
+    ok = ok && std::fscanf(f, "%f", &margin) == 1 && std::fscanf(f, "%d", &knots) == 1 && knots >= 2 &&
+         knots <= qmx::kMaxCurve;
+    for (int k = 0; ok && k < knots; ++k) ok = std::fscanf(f, "%f %f %f", &kx[k], &ky[2 * k], &ky[2 * k + 1]) == 3;
     std::fclose(f);
     if (!ok) {
         std::fprintf(stderr, "libquest_mi355x: qmxPlannerLoad: %s is not a planner file\n", path);
@@ -228,7 +259,10 @@ qmxPlanner *qmxPlannerLoad(const char *path) {
     qmxPlannerSetParams(p, w.data());
     qmxPlannerSetNormalisation(p, mean, scale);
     qmxPlannerSetMargin(p, margin);
-    qmxPlannerSetSizeExponents(p, beta);
+    if (!qmxPlannerSetSizeCurve(p, knots, kx, ky)) {
+        qmxPlannerDestroy(p);
+        return nullptr;
+    }
     return p;
 }
 

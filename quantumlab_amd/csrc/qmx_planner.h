@@ -18,6 +18,7 @@ constexpr int kOffB2 = kOffW2 + kHidden * kHidden;
 constexpr int kOffW3 = kOffB2 + kHidden;
 constexpr int kOffB3 = kOffW3 + kHidden * kOutputs;
 constexpr int kNumParams = kOffB3 + kOutputs;  // 5378
+constexpr int kMaxCurve = 16;                   // size-curve knots
 
 struct PlannerDev {
     float *master = nullptr;  // FP32 parameters (Adam updates these)
@@ -29,9 +30,22 @@ struct PlannerDev {
 
 struct PlannerHost {
     float mean[kFeatures], scale[kFeatures];
-    float margin = 0.f;                    // log2 safety margin added to predictions
-    float sizeExp[kOutputs] = {0.25f, 0.125f};  // log2 r* grows as sizeExp * log2(n)
+    float margin = 0.f;  // log2 safety margin added to predictions
+    // size curve: mean log2 r*_o at knots log2(n) (ascending), piecewise linear,
+    // extended past the ends with the end segments' slopes; default: log2(n)/(2o)
+    int knots = 2;
+    float knotLog2n[kMaxCurve] = {0.f, 1.f};
+    float knotLog2r[kMaxCurve][kOutputs] = {{0.f, 0.f}, {0.25f, 0.125f}};
 };
+
+// mean log2 r*_o of the size curve at log2(n)
+inline double sizeCurve(const PlannerHost &h, int o, double x) {
+    int k = 0;
+    while (k < h.knots - 2 && x > h.knotLog2n[k + 1]) ++k;
+    const double x0 = h.knotLog2n[k], x1 = h.knotLog2n[k + 1];
+    const double y0 = h.knotLog2r[k][o], y1 = h.knotLog2r[k + 1][o];
+    return y0 + (y1 - y0) * (x - x0) / (x1 - x0);
+}
 
 }  // namespace qmx
 

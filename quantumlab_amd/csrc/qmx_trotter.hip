@@ -205,7 +205,37 @@ __global__ void __launch_bounds__(kBlock) kTrotterLadder(const BatchJob *jobs, c
     if (threadIdx.x == 0) pass[blockIdx.x] = passed;
 }
 
-// host-side packing shared by both entry points
+// 1 - |<psi_r|psi_4r>|^2 for the job's r (BatchJob::reps); psi_r waits in a
+// global slot of the register's size while psi_4r is evolved in LDS
+__global__ void __launch_bounds__(kBlock) kTrotterInfidelity(const BatchJob *jobs, const PTerm *terms,
+                                                             const Stage *stages, const double *theta,
+                                                             const double *phiA, double2 *slot, double *infid) {
+    extern __shared__ double2 st[];
+    __shared__ double2 f0[kMaxLdsQubits], f1[kMaxLdsQubits];
+    const BatchJob j = jobs[blockIdx.x];
+    const int dim = 1 << j.n;
+    double2 *keep = slot + j.outOff;
+    loadFactors(j, theta, phiA, f0, f1);
+    initProduct(st, j.n, f0, f1);
+    __syncthreads();
+    evolveLds(st, j, terms, stages, j.reps);
+    for (int i = threadIdx.x; i < dim; i += kBlock) keep[i] = st[i];
+    __syncthreads();
+    initProduct(st, j.n, f0, f1);
+    __syncthreads();
+    evolveLds(st, j, terms, stages, 4 * j.reps);
+    Acc acc[2];
+    for (int i = threadIdx.x; i < dim; i += kBlock) {
+        const double2 a = keep[i], b = st[i];
+        acc[0].add(a.x * b.x + a.y * b.y);
+        acc[1].add(a.x * b.y - a.y * b.x);
+    }
+    double ip[2];
+    blockReduce<2>(acc, ip);
+    if (threadIdx.x == 0) infid[blockIdx.x] = 1.0 - (ip[0] * ip[0] + ip[1] * ip[1]);
+}
+
+// host-side packing shared by the entry points
 struct Packed {
     std::vector<BatchJob> jobs;
     std::vector<PTerm> terms;
@@ -322,6 +352,56 @@ int qmxTrotterBatchEvolve(int numJobs, const PauliHamil *hamils, const qreal *ti
         outImag[i] = h[i].y;
     }
     for (void *p : {(void *)dJobs, (void *)dTerms, (void *)dStages, (void *)dTh, (void *)dPh, (void *)dOut}) devFree(p);
+    return 1;
+}
+
+int qmxTrotterBatchInfidelity(int numJobs, const PauliHamil *hamils, const qreal *times, const qreal *theta,
+                              const qreal *phi, const int *orders, const int *reps, qreal *infidelity) {
+    using namespace qmx;
+    runtimeInit();
+    Packed pk;
+    if (numJobs < 1 || !pack(numJobs, hamils, times, theta, phi, orders, pk, "qmxTrotterBatchInfidelity")) return 0;
+    int allN = 1;
+    for (int j = 0; j < numJobs; ++j) {
+        if (reps[j] < 1 || reps[j] > (1 << 28)) {
+            std::fprintf(stderr, "libquest_mi355x: qmxTrotterBatchInfidelity: job %d: reps must be in 1..2^28\n", j);
+            return 0;
+        }
+        pk.jobs[j].reps = reps[j];
+        allN = std::max(allN, pk.jobs[j].n);
+    }
+    if (!allowLds(reinterpret_cast<const void *>(kTrotterInfidelity), (size_t)(1u << allN) * sizeof(double2),
+                  "qmxTrotterBatchInfidelity"))
+        return 0;
+    PTerm *dTerms = upload(pk.terms);
+    Stage *dStages = upload(pk.stages);
+    double *dTh = upload(pk.theta), *dPh = upload(pk.phi);
+    const size_t kSlotBudget = 1ull << 30;
+    int first = 0;
+    while (first < numJobs) {
+        long long amps = 0;
+        int last = first, maxN = 1;
+        while (last < numJobs) {
+            const long long need = 1ll << pk.jobs[last].n;
+            if (last > first && (size_t)(amps + need) * sizeof(double2) > kSlotBudget) break;
+            pk.jobs[last].outOff = amps;
+            amps += need;
+            maxN = std::max(maxN, pk.jobs[last].n);
+            ++last;
+        }
+        const int count = last - first;
+        std::vector<BatchJob> part(pk.jobs.begin() + first, pk.jobs.begin() + last);
+        BatchJob *dJobs = upload(part);
+        double2 *dSlot = static_cast<double2 *>(devAlloc((size_t)amps * sizeof(double2)));
+        double *dInf = static_cast<double *>(devAlloc((size_t)count * sizeof(double)));
+        hipLaunchKernelGGL(kTrotterInfidelity, dim3(count), dim3(kBlock), (size_t)(1u << maxN) * sizeof(double2),
+                           hstream(), dJobs, dTerms, dStages, dTh, dPh, dSlot, dInf);
+        checkLaunch("qmxTrotterBatchInfidelity");
+        deviceToHost(infidelity + first, dInf, (size_t)count * sizeof(double));
+        for (void *p : {(void *)dJobs, (void *)dSlot, (void *)dInf}) devFree(p);
+        first = last;
+    }
+    for (void *p : {(void *)dTerms, (void *)dStages, (void *)dTh, (void *)dPh}) devFree(p);
     return 1;
 }
 

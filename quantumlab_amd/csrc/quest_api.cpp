@@ -158,30 +158,53 @@ void pauliRot(const Qureg &q, uint64_t ctrl, const int *targs, const enum pauliO
 }
 
 // ---- Trotter-Suzuki (QuEST_common.c:762-844) --------------------------------
-void exponentiatedHamil(const Qureg &q, const PauliHamil &h, double fac, bool reverse) {
-    std::vector<int> targs(h.numQubits);
-    for (int i = 0; i < h.numQubits; ++i) targs[i] = i;
-    for (int i = 0; i < h.numSumTerms; ++i) {
-        int t = reverse ? h.numSumTerms - 1 - i : i;
-        double angle = 2 * fac * h.termCoeffs[t];
-        pauliRot(q, 0, targs.data(), &h.pauliCodes[(size_t)t * h.numQubits], h.numQubits, angle);
+// QuEST's recursion, recording the rotations of one repetition in QuEST's order
+// (density matrices: each rotation on the rows, then its conjugate on the
+// columns); the repetitions then run as fused sweeps (applyPauliSequence).
+void pushPauliTerm(std::vector<qmx::PauliExp> &seq, const Qureg &q, const enum pauliOpType *codes, int n,
+                   double angle) {
+    uint64_t x = 0, z = 0;
+    int ny = 0;
+    for (int t = 0; t < n; ++t) {
+        const uint64_t b = 1ull << t;
+        switch (codes[t]) {
+            case PAULI_X: x |= b; break;
+            case PAULI_Y: x |= b; z |= b; ++ny; break;
+            case PAULI_Z: z |= b; break;
+            default: break;
+        }
+    }
+    seq.push_back(qmx::PauliExp{x, z, ny, angle});
+    if (q.isDensityMatrix) {
+        const int s = q.numQubitsRepresented;
+        seq.push_back(qmx::PauliExp{x << s, z << s, ny, (ny & 1) ? angle : -angle});
     }
 }
 
-void symmetrizedTrotter(const Qureg &q, const PauliHamil &h, double time, int order) {
+void exponentiatedHamil(std::vector<qmx::PauliExp> &seq, const Qureg &q, const PauliHamil &h, double fac,
+                        bool reverse) {
+    for (int i = 0; i < h.numSumTerms; ++i) {
+        int t = reverse ? h.numSumTerms - 1 - i : i;
+        double angle = 2 * fac * h.termCoeffs[t];
+        pushPauliTerm(seq, q, &h.pauliCodes[(size_t)t * h.numQubits], h.numQubits, angle);
+    }
+}
+
+void symmetrizedTrotter(std::vector<qmx::PauliExp> &seq, const Qureg &q, const PauliHamil &h, double time,
+                        int order) {
     if (order == 1) {
-        exponentiatedHamil(q, h, time, false);
+        exponentiatedHamil(seq, q, h, time, false);
     } else if (order == 2) {
-        exponentiatedHamil(q, h, time / 2., false);
-        exponentiatedHamil(q, h, time / 2., true);
+        exponentiatedHamil(seq, q, h, time / 2., false);
+        exponentiatedHamil(seq, q, h, time / 2., true);
     } else {
         double p = 1. / (4 - std::pow(4, 1. / (order - 1)));
         int lower = order - 2;
-        symmetrizedTrotter(q, h, p * time, lower);
-        symmetrizedTrotter(q, h, p * time, lower);
-        symmetrizedTrotter(q, h, (1 - 4 * p) * time, lower);
-        symmetrizedTrotter(q, h, p * time, lower);
-        symmetrizedTrotter(q, h, p * time, lower);
+        symmetrizedTrotter(seq, q, h, p * time, lower);
+        symmetrizedTrotter(seq, q, h, p * time, lower);
+        symmetrizedTrotter(seq, q, h, (1 - 4 * p) * time, lower);
+        symmetrizedTrotter(seq, q, h, p * time, lower);
+        symmetrizedTrotter(seq, q, h, p * time, lower);
     }
 }
 
@@ -1197,7 +1220,9 @@ void applyTrotterCircuit(Qureg qureg, PauliHamil hamil, qreal time, int order, i
     qv::pauliHamil(hamil, __func__);
     qv::matchingQuregPauliHamilDims(qureg, hamil, __func__);
     if (time == 0) return;
-    for (int r = 0; r < reps; ++r) symmetrizedTrotter(qureg, hamil, time / reps, order);
+    std::vector<qmx::PauliExp> seq;
+    symmetrizedTrotter(seq, qureg, hamil, time / reps, order);
+    qmx::applyPauliSequence(D(qureg), seq.data(), (int)seq.size(), reps);
     API_CATCH_VOID
 }
 

@@ -61,6 +61,44 @@ def batch_evolve(qe, insts, orders, reps):
     return out
 
 
+def batch_infidelity(qe, insts, orders, reps) -> np.ndarray:
+    """1 - |<psi_r|psi_4r>|^2 per instance (FP64, batched in LDS)."""
+    hb = HamilBatch(insts)
+    o = np.ascontiguousarray(orders, dtype=np.int32)
+    r = np.ascontiguousarray(reps, dtype=np.int32)
+    out = np.zeros(len(insts))
+    ok = qe.lib.qmxTrotterBatchInfidelity(len(hb), hb.hamils, _f64p(hb.times), _f64p(hb.theta), _f64p(hb.phi),
+                                          o.ctypes.data_as(C.POINTER(C.c_int)),
+                                          r.ctypes.data_as(C.POINTER(C.c_int)), _f64p(out))
+    if not ok:
+        raise ValueError("qmxTrotterBatchInfidelity rejected the batch")
+    return out
+
+
+def cheapest_reps(qe, insts, order: int, eps: float, max_log2: int = 12):
+    """The smallest r (integer) with 1 - |<psi_r|psi_4r>|^2 <= eps for each
+    instance: the ladder r = 1, 2, 4, ... brackets it, bisection (batched)
+    finds it; None where the ladder never passes. (The first passing r of the
+    bisection, should the criterion not be monotonic inside the bracket.)"""
+    inf, pas = batch_ladder(qe, insts, [order] * len(insts), max_log2, eps)
+    lo = np.zeros(len(insts), dtype=np.int64)
+    hi = np.zeros(len(insts), dtype=np.int64)
+    for j, k in enumerate(pas):
+        if k >= 0:
+            hi[j] = 1 << int(k)
+            lo[j] = (1 << int(k)) // 2 if k > 0 else 0
+    while True:
+        act = np.nonzero((pas >= 0) & (hi - lo > 1))[0]
+        if act.size == 0:
+            break
+        mid = (lo[act] + hi[act]) // 2
+        v = batch_infidelity(qe, [insts[j] for j in act], [order] * act.size, mid)
+        ok = v <= eps
+        hi[act[ok]] = mid[ok]
+        lo[act[~ok]] = mid[~ok]
+    return [int(hi[j]) if pas[j] >= 0 else None for j in range(len(insts))]
+
+
 def batch_ladder(qe, insts, orders, max_log2: int, eps: float):
     """Infidelity ladders 1 - |<psi_r|psi_4r>|^2 for r = 1, 2, 4, ... (NaN past
     the first rung at or below eps) and the index of that rung (-1 if none)."""
@@ -117,14 +155,19 @@ class Planner:
     def margin(self) -> float:
         return float(self.lib.qmxPlannerGetMargin(self.h))
 
-    def set_size_exponents(self, beta):
-        b, bp = _f32(beta)
-        self.lib.qmxPlannerSetSizeExponents(self.h, bp)
+    def set_size_curve(self, log2n, log2reps):
+        x, xp = _f32(log2n)
+        y, yp = _f32(log2reps, (-1, 2))
+        if not self.lib.qmxPlannerSetSizeCurve(self.h, len(x), xp, yp):
+            raise ValueError("invalid size curve")
 
-    def size_exponents(self) -> np.ndarray:
-        b = np.zeros(NUM_OUTPUTS, dtype=np.float32)
-        self.lib.qmxPlannerGetSizeExponents(self.h, b.ctypes.data_as(C.POINTER(C.c_float)))
-        return b
+    def size_curve(self):
+        k = self.lib.qmxPlannerGetSizeCurve(self.h, None, None)
+        x = np.zeros(k, dtype=np.float32)
+        y = np.zeros((k, 2), dtype=np.float32)
+        self.lib.qmxPlannerGetSizeCurve(self.h, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                        y.ctypes.data_as(C.POINTER(C.c_float)))
+        return x, y
 
     def predict(self, X) -> np.ndarray:
         X, xp = _f32(X)
@@ -184,25 +227,27 @@ def log2_reps_from_ladders(inf2, inf4, eps: float):
     return L
 
 
-def fit_size_exponents(L, sizes) -> np.ndarray:
-    """beta_o: least-squares slope of the per-size mean of log2 r*_o against
-    log2(n) (defaults 1/(2o) with fewer than two sizes)."""
+def fit_size_curve(L, sizes):
+    """Knots of the size curve: log2 n of each distinct size and the mean
+    log2 r*_o (orders 2, 4) over its labelled instances."""
     sizes = np.asarray(sizes)
     uniq = np.unique(sizes)
-    beta = np.array([0.25, 0.125])
-    if len(uniq) < 2:
-        return beta
-    x = np.log2(uniq)
-    for c in range(2):
-        m = np.array([np.nanmean(L[sizes == n, c]) for n in uniq])
-        beta[c] = np.polyfit(x, m, 1)[0]
-    return beta
+    x = np.log2(uniq.astype(np.float64))
+    y = np.array([[np.nanmean(L[sizes == n, c]) for c in range(2)] for n in uniq])
+    return x, y
 
 
-def size_free_targets(L, sizes, beta):
-    """Targets y_o = log2 r*_o - beta_o log2(n) and their masks."""
-    sizes = np.asarray(sizes, dtype=np.float64)
-    T = L - np.log2(sizes)[:, None] * np.asarray(beta)[None, :]
+def size_curve(x, y, log2n):
+    """The library's piecewise-linear curve (end slopes extended), per order."""
+    log2n = np.atleast_1d(np.asarray(log2n, dtype=np.float64))
+    k = np.clip(np.searchsorted(x, log2n, side="left") - 1, 0, len(x) - 2)
+    x0, x1 = x[k], x[k + 1]
+    return y[k] + (y[k + 1] - y[k]) * ((log2n - x0) / (x1 - x0))[:, None]
+
+
+def size_free_targets(L, sizes, x, y):
+    """Targets y_o = log2 r*_o - curve_o(n) and their masks."""
+    T = L - size_curve(x, y, np.log2(np.asarray(sizes, dtype=np.float64)))
     M = np.isfinite(T).astype(np.float32)
     return np.nan_to_num(T).astype(np.float32), M
 

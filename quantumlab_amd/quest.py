@@ -270,6 +270,9 @@ _EXT_SIGS = {
     "qmxTrotterBatchEvolve": (C.c_int, [C.c_int, C.POINTER(PauliHamil), C.POINTER(qreal), C.POINTER(qreal),
                                         C.POINTER(qreal), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(qreal),
                                         C.POINTER(qreal)]),
+    "qmxTrotterBatchInfidelity": (C.c_int, [C.c_int, C.POINTER(PauliHamil), C.POINTER(qreal), C.POINTER(qreal),
+                                            C.POINTER(qreal), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(qreal)]),
     "qmxTrotterBatchLadder": (C.c_int, [C.c_int, C.POINTER(PauliHamil), C.POINTER(qreal), C.POINTER(qreal),
                                         C.POINTER(qreal), C.POINTER(C.c_int), C.c_int, qreal, C.POINTER(qreal),
                                         C.POINTER(C.c_int)]),
@@ -284,8 +287,9 @@ _EXT_SIGS = {
     "qmxPlannerGetNormalisation": (None, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "qmxPlannerSetMargin": (None, [C.c_void_p, C.c_float]),
     "qmxPlannerGetMargin": (C.c_float, [C.c_void_p]),
-    "qmxPlannerSetSizeExponents": (None, [C.c_void_p, C.POINTER(C.c_float)]),
-    "qmxPlannerGetSizeExponents": (None, [C.c_void_p, C.POINTER(C.c_float)]),
+    "qmxPlannerSetSizeCurve": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "qmxPlannerGetSizeCurve": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "qmxPlannerSizeCurve": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
     "qmxPlannerPredict": (None, [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
     "qmxPlannerGradient": (C.c_double, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                         C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
