@@ -27,6 +27,12 @@ iteration) in swap-tests/s, with its distance error against exact FP64 inner
 products, assignment agreement with the FP64 run, and its own CPU baseline:
 the reference's gate-by-gate swap-test circuits on QuEST's CPU backend.
 
+`trotter`: Hamiltonian simulation with the learned Trotter planner at 24
+qubits (FP32): solve time (features + network + one applyTrotterCircuit at
+the planner's order and repetitions) against plain step doubling at fixed
+order 2 and 4, the learned result's infidelity against the 4x-finer FP64
+run, and QuEST's CPU backend per Pauli exponential scaled to the same solves.
+
 `density`: the noisy brick circuit (60 layers, dephasing + depolarising +
 damping after every gate on its qubits) on a 14-qubit density matrix in FP64
 and FP32, with purity, per-qubit P(0), fidelity with the noiseless run, the
@@ -205,6 +211,101 @@ def cpu_density_baseline(n_cpu, layers, seed):
                       f"OpenMP {threads} threads; {dt:.2f} s"}
 
 
+def run_trotter(qe, n, count, seed, eps=1e-4):
+    """Hamiltonian simulation at n qubits (FP32 registers): per instance, the
+    learned planner's (order, reps) and one applyTrotterCircuit, timed from the
+    Hamiltonian to the evolved state (features, network, evolution); its
+    normalised infidelity against the 4x-finer FP64 run at the same order;
+    and plain step doubling at fixed order 2 and 4, timed the same way."""
+    import ctypes as C
+
+    import numpy as np
+    from quantumlab_amd import hamiltonians as H
+    from quantumlab_amd import planner as PL
+    from quantumlab_amd.quest import QMX_PREC_FP32
+    net = PL.Planner(qe, path=os.path.join(ROOT, "quantumlab_amd", "data", "trotter_planner.txt"))
+    rows = []
+    for idx in range(count):
+        inst = H.trotter_instance(n, seed, idx)
+        hb = H.HamilBatch([inst])
+        q = qe.qureg(n, precision=QMX_PREC_FP32)
+        H.prepare(qe, q, inst)
+        qe.qmxSync()
+        o, r = C.c_int(), C.c_int()
+        t0 = time.perf_counter()
+        qe.lib.qmxApplyTrotterLearned(q, hb[0], inst.time, net.h, C.byref(o), C.byref(r))
+        qe.qmxSync()
+        t_learned = time.perf_counter() - t0
+        ref = qe.qureg(n)
+        H.prepare(qe, ref, inst)
+        qe.lib.applyTrotterCircuit(ref, hb[0], inst.time, o.value, 4 * r.value)
+        ip = qe.calcInnerProduct(ref, q)
+        infid = 1 - (ip.real ** 2 + ip.imag ** 2) / (qe.calcTotalProb(ref) * qe.calcTotalProb(q))
+        qe.destroy(ref)
+        sd = {}
+        for order in (2, 4):
+            H.prepare(qe, q, inst)
+            qe.qmxSync()
+            reps = C.c_int()
+            t0 = time.perf_counter()
+            ok = qe.lib.qmxApplyTrotterStepDoubling(q, hb[0], inst.time, order, eps, 1 << 14, C.byref(reps))
+            qe.qmxSync()
+            sd[order] = (time.perf_counter() - t0, reps.value, ok)
+        qe.destroy(q)
+        rows.append((inst, o.value, r.value, t_learned, infid, sd))
+        log(f"trotter {idx}: {inst.model} {inst.terms} terms t={inst.time:.2f}: learned order {o.value} x{r.value} "
+            f"{t_learned * 1e3:.0f} ms (1-F {infid:.1e}); step doubling o2 x{sd[2][1]} {sd[2][0] * 1e3:.0f} ms, "
+            f"o4 x{sd[4][1]} {sd[4][0] * 1e3:.0f} ms")
+    net.close()
+    tl = np.array([x[3] for x in rows])
+    t2 = np.array([x[5][2][0] for x in rows])
+    t4 = np.array([x[5][4][0] for x in rows])
+    inf = np.array([x[4] for x in rows])
+    exps = sum(H.stages(x[1]) * x[2] * x[0].terms for x in rows)
+    return {"metric": "Trotter solve time at F>=1-1e-4", "unit": "ms", "higher_is_better": False,
+            "workload": f"{count} seeded Hamiltonians (TFIM/Heisenberg chains + 32 random 2-4-local strings) on "
+                        f"{n} qubits, random product states, t ~ U(0.5, 4); FP32 registers",
+            "qubits": n, "instances": count, "dtype": "f32",
+            "learned_ms_mean": round(float(tl.mean() * 1e3), 2),
+            "step_doubling_o2_ms_mean": round(float(t2.mean() * 1e3), 2),
+            "step_doubling_o4_ms_mean": round(float(t4.mean() * 1e3), 2),
+            "speedup_vs_step_doubling_o2": round(float(t2.sum() / tl.sum()), 3),
+            "speedup_vs_step_doubling_o4": round(float(t4.sum() / tl.sum()), 3),
+            "speedup_vs_best_fixed_order": round(float(min(t2.sum(), t4.sum()) / tl.sum()), 3),
+            "success_rate": float(np.mean(inf <= eps)), "max_infidelity_vs_4x_finer_fp64": float(inf.max()),
+            "step_doubling_met_eps": bool(all(x[5][2][2] and x[5][4][2] for x in rows)),
+            "pauli_exponentials_per_s_learned": round(exps / tl.sum(), 1),
+            "choices": [[x[1], x[2]] for x in rows],
+            "step_doubling_reps": [[x[5][2][1], x[5][4][1]] for x in rows]}, rows
+
+
+def cpu_trotter_baseline(rows, n):
+    """QuEST's CPU backend: one order-1 repetition (every term once) of the
+    first instance at n qubits (FP64), scaled to each learned choice."""
+    from quantumlab_amd import hamiltonians as H
+    from quantumlab_amd.quest import QuEST
+    ref = os.path.join(ROOT, "oracle", "_ref", "libquest_ref.so")
+    port = os.path.join(ROOT, "oracle", "build", "libcpuref.so")
+    kind, path = ("reference", ref) if os.path.exists(ref) else ("port", port)
+    lib = QuEST(path)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    inst = rows[0][0]
+    hb = H.HamilBatch([inst])
+    q = lib.qureg(n)
+    H.prepare(lib, q, inst)
+    t0 = time.perf_counter()
+    lib.lib.applyTrotterCircuit(q, hb[0], inst.time, 1, 1)
+    dt = time.perf_counter() - t0
+    lib.destroy(q)
+    per_exp = dt / inst.terms
+    solve = [H.stages(x[1]) * x[2] * x[0].terms * per_exp for x in rows]
+    return {"value": round(1e3 * sum(solve) / len(solve), 1), "unit": "ms", "cores": threads, "kind": kind,
+            "sample": f"one order-1 repetition ({inst.terms} Pauli exponentials) of the first instance at {n} "
+                      f"qubits, FP64, {dt:.2f} s; the solve time is that per-exponential cost times the learned "
+                      f"choices' exponentials; {'QuEST 3.4 CPU backend compiled from the reference sources' if kind == 'reference' else 'C restatement of it'}, "
+                      f"OpenMP {threads} threads"}
+
+
 def exact_p0(data, cents):
     """P(ancilla = 0) = (1 + <a|b>^2)/2 from FP64 numpy inner products of the
     normalised points and the normalised truncated centroids."""
@@ -336,6 +437,9 @@ def main():
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--no-swap", action="store_true")
     ap.add_argument("--no-density", action="store_true")
+    ap.add_argument("--no-trotter", action="store_true")
+    ap.add_argument("--trotter-qubits", type=int, default=24)
+    ap.add_argument("--trotter-count", type=int, default=8)
     ap.add_argument("--density-qubits", type=int, default=14)
     ap.add_argument("--density-layers", type=int, default=60)
     ap.add_argument("--cpu-density-qubits", type=int, default=13)
@@ -431,6 +535,13 @@ def main():
     if not args.no_density:
         result["density"] = run_density(qe, args.density_qubits, args.density_layers, args.seed)
 
+    trotter_rows = None
+    if not args.no_trotter:
+        result["trotter"], trotter_rows = run_trotter(qe, args.trotter_qubits, args.trotter_count, args.seed)
+        tr = result["trotter"]
+        log(f"trotter: learned {tr['learned_ms_mean']} ms vs step doubling o2 {tr['step_doubling_o2_ms_mean']} ms, "
+            f"o4 {tr['step_doubling_o4_ms_mean']} ms; success {tr['success_rate']:.2f}")
+
     swap_data = None
     if not args.no_swap:
         result["swap_tests"], swap_data = run_swap_tests(qe, args.swap_points, args.swap_centroids,
@@ -447,6 +558,9 @@ def main():
         if "density" in result:
             result["density"]["cpu_baseline"] = cpu_density_baseline(args.cpu_density_qubits, 2, args.seed)
             log(f"cpu density baseline: {result['density']['cpu_baseline']['value']} GB/s")
+        if trotter_rows is not None:
+            result["trotter"]["cpu_baseline"] = cpu_trotter_baseline(trotter_rows, args.trotter_qubits)
+            log(f"cpu trotter baseline: {result['trotter']['cpu_baseline']['value']} ms per solve")
         if swap_data is not None:
             result["swap_tests"]["cpu_baseline"] = cpu_swap_baseline(swap_data)
             log(f"cpu swap-test baseline: {result['swap_tests']['cpu_baseline']['value']} swap-tests/s")
