@@ -424,45 +424,10 @@ def cpu_swap_baseline(data, seconds_per_size=2.0):
             "max_p0_error_vs_closed_form": err}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--qubits", type=int, default=32)
-    ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--fuse-qubits", type=int, default=5)
-    ap.add_argument("--cpu-qubits", type=int, default=27)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fused", action="store_true")
-    ap.add_argument("--no-swap", action="store_true")
-    ap.add_argument("--no-density", action="store_true")
-    ap.add_argument("--no-trotter", action="store_true")
-    ap.add_argument("--trotter-qubits", type=int, default=24)
-    ap.add_argument("--trotter-count", type=int, default=8)
-    ap.add_argument("--density-qubits", type=int, default=14)
-    ap.add_argument("--density-layers", type=int, default=60)
-    ap.add_argument("--cpu-density-qubits", type=int, default=13)
-    ap.add_argument("--swap-points", type=int, default=16384)
-    ap.add_argument("--swap-centroids", type=int, default=16)
-    ap.add_argument("--swap-iterations", type=int, default=10)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the state-vector "
-              f"path here has no multi-GPU sharding; see DESIGN.md for how it would shard)", file=sys.stderr)
-        sys.exit(2)
-
+def statevec_leg(qe, args, n, N):
+    """The headline: per-gate FP64 brick circuit, value, roofline."""
     import numpy as np
-    import quantumlab_amd
     from quantumlab_amd import workloads as W
-
-    qe = quantumlab_amd.load()
-    buf = __import__("ctypes").create_string_buffer(512)
-    qe.qmxDeviceInfo(buf, 512)
-    log(buf.value.decode())
-    n = args.qubits
-    N = 1 << n
-
     q64, plans, timed, wall, ev_ms, cu = run_fp64_per_gate(qe, n, args.steps, args.warmup, args.seed)
     alg_bytes = W.gate_bytes(timed, n, 16)
     value = alg_bytes / wall / 1e9
@@ -513,8 +478,59 @@ def main():
     }
     log(f"fp64 per-gate: {value:.1f} GB/s, {result['gates_per_s']:.1f} gates/s, compactUnitary "
         f"{cu_ms.mean():.3f} ms avg -> {achieved:.1f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of peak)")
+    return result, q64, plans, timed, wall
 
-    if not args.no_fused:
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=32)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--fuse-qubits", type=int, default=5)
+    ap.add_argument("--cpu-qubits", type=int, default=27)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fused", action="store_true")
+    ap.add_argument("--no-swap", action="store_true")
+    ap.add_argument("--no-density", action="store_true")
+    ap.add_argument("--no-trotter", action="store_true")
+    ap.add_argument("--trotter-qubits", type=int, default=24)
+    ap.add_argument("--trotter-count", type=int, default=8)
+    ap.add_argument("--density-qubits", type=int, default=14)
+    ap.add_argument("--density-layers", type=int, default=60)
+    ap.add_argument("--cpu-density-qubits", type=int, default=13)
+    ap.add_argument("--swap-points", type=int, default=16384)
+    ap.add_argument("--swap-centroids", type=int, default=16)
+    ap.add_argument("--swap-iterations", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the state-vector "
+              f"path here has no multi-GPU sharding; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import quantumlab_amd
+    from quantumlab_amd import workloads as W
+
+    qe = quantumlab_amd.load()
+    buf = __import__("ctypes").create_string_buffer(512)
+    qe.qmxDeviceInfo(buf, 512)
+    log(buf.value.decode())
+    n = args.qubits
+    N = 1 << n
+
+    if args.steps > 0:
+        result, q64, plans, timed, wall = statevec_leg(qe, args, n, N)
+    else:
+        # profiling runs of the other legs only: no headline value
+        q64 = None
+        result = {"metric": METRIC, "value": None, "unit": "GB/s", "n_gpus": 1, "steps": 0,
+                  "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+                  "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+                  "config": {"workload": "state-vector leg skipped (--steps 0)", "qubits": n}}
+
+    if not args.no_fused and q64 is not None:
         q32, wall32, blocks, norm2 = run_fp32_fused(qe, n, plans, args.warmup, args.steps, args.fuse_qubits)
         ip = qe.calcInnerProduct(q64, q32)
         # normalised fidelity: norms from FP64 reductions (the FP32-stored vector
@@ -530,7 +546,8 @@ def main():
                                 "max_prob_error": float(p_err), "norm2_before_renorm": float(norm2)}
         log(f"fused fp32: {wall / wall32:.2f}x vs fp64 per-gate, 1-F={1 - fid:.3e}, max|da|={maxabs:.3e}")
         qe.destroy(q32)
-    qe.destroy(q64)
+    if q64 is not None:
+        qe.destroy(q64)
 
     if not args.no_density:
         result["density"] = run_density(qe, args.density_qubits, args.density_layers, args.seed)

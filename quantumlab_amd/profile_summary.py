@@ -50,11 +50,19 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--name", required=True)
     ap.add_argument("--trace", required=True)
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
+    ap.add_argument("--fetch", default=None)
+    ap.add_argument("--write", default=None)
     ap.add_argument("--out", default="profiles")
     ap.add_argument("--qubits", type=int, default=32)
+    ap.add_argument("--trace-cmd", default=None, help="the rocprofv3 command of the trace (kernel-trace-only summaries)")
     args = ap.parse_args()
+    if args.fetch is None:
+        # kernel durations only
+        os.makedirs(args.out, exist_ok=True)
+        with open(os.path.join(args.out, f"{args.name}_kernel_trace.md"), "w") as f:
+            f.write(f"# {args.name}: kernel durations (rocprofv3 kernel trace)\n\n")
+            f.write(trace_table(args.trace, args.trace_cmd) + "\n")
+        return
     trace_cmd = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_trace -o bench -- "
                  "python3 bench.py --steps 1 --warmup 0 --no-cpu")
     pmc_cmd = ("rocprofv3 --pmc {c} --output-format csv -d gpurun_out/prof_{d} -o bench -- "
