@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #include "qmx_device.h"
@@ -194,6 +195,98 @@ __global__ void __launch_bounds__(kBlock) kSwapTiles(const Tile *__restrict__ ti
     }
 }
 
+// FP32 plans: the same sums on the matrix cores, one wave per tile. A tile's
+// rows x0 .. x0+nx (nx a multiple of 32, or the whole register when D < 32)
+// against all columns, in 32 x 32 blocks of M[x][y] = a_x b_y + b_x a_y (twice
+// the ancilla-0 amplitude) from one v_mfma_f32_32x32x2f32 each:
+//   A (32 x 2): lane l holds A[l%32][l/32] = (l < 32 ? a : b)[x0 + l%32]
+//   B (2 x 32): lane l holds B[l/32][l%32] = (l < 32 ? b : a)[y0 + l%32]
+// so K = 2 is exactly the two products (and M is symmetric, so only blocks on
+// or above the diagonal are formed). Each of the 16 accumulator values per
+// lane is one element of the block; their squares are summed in FP32 per column
+// block (1, 2, 4 or 8 row blocks share one B operand) and in FP64 across blocks.
+constexpr int kMaxRowBlocks = 8;  // 256 rows per tile
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// one tile of NB row blocks. M is symmetric (M[x][y] = M[y][x]), so the sum
+// over all blocks is the diagonal blocks once plus the blocks above the
+// diagonal twice: row block X only meets column blocks Y >= X.
+template <int NB>
+__device__ __forceinline__ double swapTileMfma(const float *sx, const float *sy, int x0, int D, int r) {
+    const floatx16 zero = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float av[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int x = x0 + 32 * k + r;
+        av[k] = x < D ? sx[x] : 0.f;
+    }
+    double acc = 0.0;
+    float bv = x0 + r < D ? sy[x0 + r] : 0.f;
+    for (int y0 = x0; y0 < D; y0 += 32) {
+        const int yn = y0 + 32 + r;
+        const float bnext = yn < D ? sy[yn] : 0.f;  // next column block, loaded ahead
+        float s = 0.f;
+        if (y0 >= x0 + 32 * (NB - 1)) {
+            // every row block of the tile is at or left of this column block
+            floatx16 d[NB];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) d[k] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[k], bv, zero, 0, 0, 0);
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                float sk = 0.f;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) sk = fmaf(d[k][v], d[k][v], sk);
+                s += (y0 == x0 + 32 * k) ? sk : 2.f * sk;
+            }
+        } else {
+            // the tile's first column blocks: only row blocks X <= Y
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                if (y0 >= x0 + 32 * k) {
+                    const floatx16 d = __builtin_amdgcn_mfma_f32_32x32x2f32(av[k], bv, zero, 0, 0, 0);
+                    float sk = 0.f;
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) sk = fmaf(d[v], d[v], sk);
+                    s += (y0 == x0 + 32 * k) ? sk : 2.f * sk;
+                }
+            }
+        }
+        acc += (double)s;
+        bv = bnext;
+    }
+    return acc;
+}
+
+__global__ void __launch_bounds__(kBlock) kSwapTilesMfma(const Tile *__restrict__ tiles, int numTiles,
+                                                         const float *__restrict__ pointAE,
+                                                         const long long *__restrict__ pOff, const int *__restrict__ pQ,
+                                                         const float *__restrict__ centAE,
+                                                         const long long *__restrict__ cOff, int C, int qLo,
+                                                         double *__restrict__ partial) {
+    const int lane = threadIdx.x & 63, r = lane & 31, hi = lane >> 5;
+    const int wavesTotal = gridDim.x * (kBlock / 64);
+    for (int w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < numTiles; w += wavesTotal) {
+        const Tile t = tiles[w];
+        const int p = t.test / C, c = t.test % C;
+        const int q = pQ[p];
+        const int D = 1 << q;
+        const float *a = pointAE + pOff[p];
+        const float *b = centAE + cOff[(q - qLo) * C + c];
+        const float *sx = hi ? b : a;  // rows
+        const float *sy = hi ? a : b;  // columns
+        const int nb = (t.nx + 31) >> 5;
+        double acc;
+        switch (nb) {  // wave-uniform
+            case 1: acc = swapTileMfma<1>(sx, sy, t.x0, D, r); break;
+            case 2: acc = swapTileMfma<2>(sx, sy, t.x0, D, r); break;
+            case 4: acc = swapTileMfma<4>(sx, sy, t.x0, D, r); break;
+            default: acc = swapTileMfma<8>(sx, sy, t.x0, D, r); break;
+        }
+        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) partial[w] = acc;
+    }
+}
+
 // P0 of each test: (1/4) sum of its tiles' partials (tiles of a test are contiguous)
 __global__ void __launch_bounds__(kBlock) kSwapFinish(const double *__restrict__ partial, const int *__restrict__ tStart,
                                                       const int *__restrict__ tCount, int numTests, double *__restrict__ p0) {
@@ -223,26 +316,60 @@ __global__ void __launch_bounds__(kBlock) kAssign(const double *__restrict__ p0,
 
 // centroid means over assigned points, coordinate by coordinate (points only
 // contribute their own 2^q coordinates); empty clusters keep their centroid.
-// One workgroup per (centroid, 256-coordinate slice); deterministic order; FP64.
-__global__ void __launch_bounds__(kBlock) kCentroidMeans(const double *__restrict__ pointsV,
-                                                         const long long *__restrict__ pOff, const int *__restrict__ pQ,
-                                                         const int *__restrict__ assign, int P, int C, int maxQ,
-                                                         double *__restrict__ cent) {
-    const int slices = (1 << maxQ) / kBlock > 0 ? (1 << maxQ) / kBlock : 1;
-    const int c = blockIdx.x / slices;
-    const int j = (blockIdx.x % slices) * kBlock + threadIdx.x;
-    if (c >= C || j >= (1 << maxQ)) return;
+// Stage 1: one workgroup per (centroid, 256-coordinate slice, chunk of points)
+// writes the chunk's sum and count; stage 2 adds the chunks in order. FP64,
+// deterministic.
+constexpr int kMeanChunks = 16;
+
+__global__ void __launch_bounds__(kBlock) kCentroidPartial(const double *__restrict__ pointsV,
+                                                           const long long *__restrict__ pOff,
+                                                           const int *__restrict__ pQ, const int *__restrict__ assign,
+                                                           int P, int C, int maxQ, double *__restrict__ sums,
+                                                           int *__restrict__ counts, int *__restrict__ members) {
+    const int dim = 1 << maxQ;
+    const int slices = dim / kBlock > 0 ? dim / kBlock : 1;
+    const int chunk = blockIdx.x % kMeanChunks;
+    const int cs = blockIdx.x / kMeanChunks;
+    const int c = cs / slices;
+    const int j = (cs % slices) * kBlock + threadIdx.x;
+    const int per = (P + kMeanChunks - 1) / kMeanChunks;
+    const int p0 = chunk * per, p1 = p0 + per < P ? p0 + per : P;
+    if (c >= C) return;
     double sum = 0.0;
-    int cnt = 0, members = 0;
-    for (int p = 0; p < P; ++p) {
+    int cnt = 0, mem = 0;
+    for (int p = p0; p < p1; ++p) {
         if (assign[p] != c) continue;
-        ++members;
+        ++mem;
         if (j < (1 << pQ[p])) {
             sum += pointsV[pOff[p] + j];
             ++cnt;
         }
     }
-    if (members > 0 && cnt > 0) cent[(size_t)c * (1 << maxQ) + j] = sum / cnt;
+    if (j < dim) {
+        const size_t o = ((size_t)chunk * C + c) * dim + j;
+        sums[o] = sum;
+        counts[o] = cnt;
+    }
+    if (threadIdx.x == 0 && (cs % slices) == 0) members[chunk * C + c] = mem;
+}
+
+__global__ void __launch_bounds__(kBlock) kCentroidCombine(const double *__restrict__ sums, const int *__restrict__ counts,
+                                                           const int *__restrict__ members, int C, int maxQ,
+                                                           double *__restrict__ cent) {
+    const int dim = 1 << maxQ;
+    const int slices = dim / kBlock > 0 ? dim / kBlock : 1;
+    const int c = blockIdx.x / slices;
+    const int j = (blockIdx.x % slices) * kBlock + threadIdx.x;
+    if (c >= C || j >= dim) return;
+    double sum = 0.0;
+    int cnt = 0, mem = 0;
+    for (int k = 0; k < kMeanChunks; ++k) {
+        const size_t o = ((size_t)k * C + c) * dim + j;
+        sum += sums[o];
+        cnt += counts[o];
+        mem += members[k * C + c];
+    }
+    if (mem > 0 && cnt > 0) cent[(size_t)c * dim + j] = sum / cnt;
 }
 
 }  // namespace
@@ -259,6 +386,8 @@ struct SwapPlanImpl {
     int *pQ = nullptr, *cQ = nullptr, *assign = nullptr, *tStart = nullptr, *tCount = nullptr;
     Tile *tiles = nullptr;
     double *partial = nullptr, *p0 = nullptr;
+    double *meanSums = nullptr;
+    int *meanCounts = nullptr, *meanMembers = nullptr;
     int numTiles = 0;
     int rowsPerTile[kMaxQ + 1] = {0};
     int grid = 0;
@@ -269,19 +398,34 @@ struct SwapPlanImpl {
 
 namespace {
 
+// the swap-test sums of every tile: matrix cores for FP32 registers, FP64 on the VALU
+template <typename T>
+void launchSwapTiles(SwapPlanImpl *s) {
+    if constexpr (std::is_same<T, float>::value)
+        hipLaunchKernelGGL(kSwapTilesMfma, dim3(s->grid), dim3(kBlock), 0, hstream(), s->tiles, s->numTiles,
+                           (const float *)s->pointAE, s->pOff, s->pQ, (const float *)s->centAE, s->cOff, s->C, s->qLo,
+                           s->partial);
+    else
+        hipLaunchKernelGGL(kSwapTiles<T>, dim3(s->grid), dim3(kBlock), 0, hstream(), s->tiles, s->numTiles,
+                           (const T *)s->pointAE, s->pOff, s->pQ, (const T *)s->centAE, s->cOff, s->C, s->qLo,
+                           s->partial);
+}
+
 template <typename T>
 void launchIteration(SwapPlanImpl *s) {
     const int numTests = s->P * s->C;
     const int numCent = (s->qHi - s->qLo + 1) * s->C;
-    hipLaunchKernelGGL(kSwapTiles<T>, dim3(s->grid), dim3(kBlock), 0, hstream(), s->tiles, s->numTiles,
-                       (const T *)s->pointAE, s->pOff, s->pQ, (const T *)s->centAE, s->cOff, s->C, s->qLo, s->partial);
+    launchSwapTiles<T>(s);
     hipLaunchKernelGGL(kSwapFinish, dim3((numTests + kBlock - 1) / kBlock), dim3(kBlock), 0, hstream(), s->partial,
                        s->tStart, s->tCount, numTests, s->p0);
     hipLaunchKernelGGL(kAssign, dim3((s->P + kBlock - 1) / kBlock), dim3(kBlock), 0, hstream(), s->p0, s->P, s->C,
                        s->assign);
     const int slices = std::max(1, (1 << s->maxQ) / kBlock);
-    hipLaunchKernelGGL(kCentroidMeans, dim3(s->C * slices), dim3(kBlock), 0, hstream(), (const double *)s->pointsV,
-                       s->pOff, s->pQ, s->assign, s->P, s->C, s->maxQ, (double *)s->cent);
+    hipLaunchKernelGGL(kCentroidPartial, dim3(s->C * slices * kMeanChunks), dim3(kBlock), 0, hstream(),
+                       (const double *)s->pointsV, s->pOff, s->pQ, s->assign, s->P, s->C, s->maxQ, s->meanSums,
+                       s->meanCounts, s->meanMembers);
+    hipLaunchKernelGGL(kCentroidCombine, dim3(s->C * slices), dim3(kBlock), 0, hstream(), s->meanSums, s->meanCounts,
+                       s->meanMembers, s->C, s->maxQ, (double *)s->cent);
     hipLaunchKernelGGL(kCentroidInputs, dim3(numCent), dim3(kBlock), 0, hstream(), (const double *)s->cent, s->maxQ,
                        s->C, s->qLo, s->qHi, s->cOff, (double *)s->centIn);
     hipLaunchKernelGGL(kAEState<T>, dim3(numCent), dim3(kBlock), s->aeSmem, hstream(), (const double *)s->centIn,
@@ -308,7 +452,8 @@ void buildTiles(SwapPlanImpl *s) {
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return s->hQ[x] > s->hQ[y]; });
     for (int p : order) {
         const int D = 1 << s->hQ[p];
-        const int rows = std::min(std::min(D, kMaxTileRows), s->rowsPerTile[s->hQ[p]]);
+        const int rows = std::min(std::min(D, s->prec == FP32 ? 32 * kMaxRowBlocks : kMaxTileRows),
+                                  s->rowsPerTile[s->hQ[p]]);
         for (int c = 0; c < s->C; ++c) {
             const int test = p * s->C + c;
             start[test] = (int)tl.size();
@@ -327,7 +472,9 @@ void buildTiles(SwapPlanImpl *s) {
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
-    s->grid = std::max(1, std::min(s->numTiles, cus * 8));
+    // FP32: one tile per wave (4 per workgroup); FP64: one tile per workgroup
+    const int perGroup = s->prec == FP32 ? kBlock / 64 : 1;
+    s->grid = std::max(1, std::min((s->numTiles + perGroup - 1) / perGroup, cus * 8));
 }
 
 void dropGraph(SwapPlanImpl *s) {
@@ -420,13 +567,20 @@ qmxSwapPlan *qmxSwapPlanCreate(int numPoints, const int *pointQubits, const doub
     s->tStart = static_cast<int *>(devAlloc((size_t)numPoints * numCentroids * sizeof(int)));
     s->tCount = static_cast<int *>(devAlloc((size_t)numPoints * numCentroids * sizeof(int)));
     s->p0 = static_cast<double *>(devAlloc((size_t)numPoints * numCentroids * sizeof(double)));
+    const size_t meanSlots = (size_t)kMeanChunks * numCentroids * (1ull << maxQubits);
+    s->meanSums = static_cast<double *>(devAlloc(meanSlots * sizeof(double)));
+    s->meanCounts = static_cast<int *>(devAlloc(meanSlots * sizeof(int)));
+    s->meanMembers = static_cast<int *>(devAlloc((size_t)kMeanChunks * numCentroids * sizeof(int)));
     hostToDevice(s->pOff, s->hPOff.data(), numPoints * sizeof(long long));
     hostToDevice(s->cOff, s->hCOff.data(), s->hCOff.size() * sizeof(long long));
     hostToDevice(s->pQ, s->hQ.data(), numPoints * sizeof(int));
     hostToDevice(s->cQ, cq.data(), cq.size() * sizeof(int));
     s->aeSmem = (size_t)3 * (1u << s->qHi) * sizeof(double);
-    // default: ~16K ancilla-0 amplitudes a tile
-    for (int q = 0; q <= kMaxQ; ++q) s->rowsPerTile[q] = std::max(1, std::min(16384 >> q, std::min(1 << q, kMaxTileRows)));
+    // default tile rows: FP32 (one wave per tile) 64 rows, or the register
+    // when smaller; FP64 (one workgroup per tile) ~16K amplitudes a tile
+    for (int q = 0; q <= kMaxQ; ++q)
+        s->rowsPerTile[q] = precision == FP32 ? std::min(1 << q, 64)
+                                              : std::max(1, std::min(16384 >> q, std::min(1 << q, kMaxTileRows)));
     QMX_DISPATCH(precision, T, {
         hipLaunchKernelGGL(kAEState<T>, dim3(numPoints), dim3(kBlock), s->aeSmem, hstream(),
                            (const double *)s->pointsV, s->pOff, s->pQ, (T *)s->pointAE, numPoints);
@@ -445,7 +599,8 @@ void qmxSwapPlanDestroy(qmxSwapPlan *plan) {
     dropGraph(s);
     for (void *p : {s->pointsV, s->pointAE, s->cent, s->centIn, s->centAE, (void *)s->pOff, (void *)s->cOff,
                     (void *)s->pQ, (void *)s->cQ, (void *)s->assign, (void *)s->tStart, (void *)s->tCount,
-                    (void *)s->tiles, (void *)s->partial, (void *)s->p0})
+                    (void *)s->tiles, (void *)s->partial, (void *)s->p0, (void *)s->meanSums,
+                    (void *)s->meanCounts, (void *)s->meanMembers})
         devFree(p);
     delete plan;
 }
@@ -456,7 +611,9 @@ int qmxSwapPlanAutotune(qmxSwapPlan *plan) {
     using namespace qmx;
     SwapPlanImpl *s = &plan->impl;
     dropGraph(s);
-    const int cands[] = {1, 4, 16, 64, 256, 1024, 4096};
+    // FP32 tiles are row blocks of 32 on the matrix cores; FP64 tiles any rows
+    const std::vector<int> cands = s->prec == FP32 ? std::vector<int>{32, 64, 128, 256}
+                                                   : std::vector<int>{1, 4, 16, 64, 256, 1024, 4096};
     int timings = 0;
     for (int q = s->qLo; q <= s->qHi; ++q) {
         bool present = false;
@@ -465,21 +622,25 @@ int qmxSwapPlanAutotune(qmxSwapPlan *plan) {
         const int D = 1 << q;
         double best = 1e30;
         int bestRows = s->rowsPerTile[q];
-        for (int rows : cands) {
-            if (rows > D || rows > kMaxTileRows || (long long)rows * D > (1ll << 22) ||
-                ((long long)rows * D < 256 && rows != D))
+        int lastRows = -1;
+        for (int rowsCand : cands) {
+            int rows = rowsCand;
+            if (s->prec == FP32) {
+                if (D < 32) rows = D;           // a register smaller than one block: one tile
+                else if (rows > D) continue;
+            } else if (rows > D || rows > kMaxTileRows || (long long)rows * D > (1ll << 22) ||
+                       ((long long)rows * D < 256 && rows != D)) {
                 continue;
+            }
+            if (rows == lastRows) continue;
+            lastRows = rows;
             // retime the whole tile list with this size's rows changed
             int saved[kMaxQ + 1];
             std::memcpy(saved, s->rowsPerTile, sizeof saved);
             s->rowsPerTile[q] = rows;
             buildTiles(s);
             timerBegin();
-            QMX_DISPATCH(s->prec, T, {
-                hipLaunchKernelGGL(kSwapTiles<T>, dim3(s->grid), dim3(kBlock), 0, hstream(), s->tiles, s->numTiles,
-                                   (const T *)s->pointAE, s->pOff, s->pQ, (const T *)s->centAE, s->cOff, s->C, s->qLo,
-                                   s->partial);
-            });
+            QMX_DISPATCH(s->prec, T, { launchSwapTiles<T>(s); });
             checkLaunch("autotune");
             const double ms = timerEndMs();
             ++timings;
