@@ -8,11 +8,12 @@
 // amplitudes, contiguous runs in memory — into LDS, apply all of the block's
 // rotations there in order, and write it back: one HBM read + write per block
 // instead of per rotation. Z parities split as parity(base & z) (one per tile
-// and rotation) xor parity(local & z_local). The per-rotation arithmetic is
-// the one-sweep kernels' (kPauliRot, kRotZ), term for term. Inside a tile the
+// and rotation) xor parity(local & z_local), and a pair's partner parity is
+// the pair's own xor parity(lx & z_local). The per-rotation arithmetic is the
+// one-sweep kernels' (kPauliRot, kRotZ), term for term. Inside a tile the
 // rotations run in passes: consecutive rotations whose X/Y masks coincide
 // (e.g. XX and YY on one bond), plus any diagonal rotations between them, are
-// applied to each amplitude pair in registers in one LDS round trip.
+// applied to each thread's amplitude pairs in registers in one LDS round trip.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -29,7 +30,9 @@ constexpr int kMaxTileBits = 12;
 struct PauliExpDev {
     uint64_t z;          // global z mask (parity of the tile base)
     unsigned lz;         // tile-local z mask
+    unsigned pxz;        // parity(lx & lz) of the pass's exchange mask: sign of the partner
     int diag;            // no X/Y factor
+    int pad;
     double c, s;         // cos(theta/2), sin(theta/2)
     double kre, kim;     // -i sin(theta/2) i^nY
 };
@@ -48,98 +51,121 @@ struct PauliTile {
     BitList fixed;            // the same positions (tile base = insert zeros there)
 };
 
-// a rotation as one tile uses it: working precision, and the parity of the
-// tile base folded into the signs (s for diagonal rotations, k otherwise)
-template <typename T>
-struct OpT {
-    unsigned lz;
-    int diag;
-    T c, s, kre, kim;
-};
-
-template <typename T>
-__device__ __forceinline__ void rotatePair(Amp<T> &vi, Amp<T> &vp, const OpT<T> &e, unsigned i, unsigned p) {
-    if (e.diag) {
-        const T fi = (__popc(i & e.lz) & 1) ? -e.s : e.s;
-        const T fp = (__popc(p & e.lz) & 1) ? -e.s : e.s;
-        vi = Amp<T>{e.c * vi.x + fi * vi.y, e.c * vi.y - fi * vi.x};
-        vp = Amp<T>{e.c * vp.x + fp * vp.y, e.c * vp.y - fp * vp.x};
-        return;
-    }
-    const T si = (__popc(p & e.lz) & 1) ? (T)-1 : (T)1;
-    const T sp = (__popc(i & e.lz) & 1) ? (T)-1 : (T)1;
-    const Amp<T> a = vi, b = vp;
-    vi = Amp<T>{e.c * a.x + si * (e.kre * b.x - e.kim * b.y), e.c * a.y + si * (e.kre * b.y + e.kim * b.x)};
-    vp = Amp<T>{e.c * b.x + sp * (e.kre * a.x - e.kim * a.y), e.c * b.y + sp * (e.kre * a.y + e.kim * a.x)};
-}
-
-constexpr int kMaxBlockOps = 128;  // rotations staged per tile (LDS)
-
-template <typename T>
-__global__ void __launch_bounds__(kBlock) kPauliBlock(Amp<T> *__restrict__ a, PauliTile tile,
+// PPT amplitude pairs per thread and pass (tiles of 2^(9 + log2 PPT) amplitudes):
+// a thread keeps its pairs in registers while every rotation of the pass is
+// applied, so a rotation's constants are read once per thread, from scalar
+// loads, with the tile base's parity folded into its signs. A workgroup walks
+// tiles w = blockIdx.x + k gridDim.x, issuing the global loads of its next
+// tile before the passes of the current one.
+template <typename T, int PPT>
+__global__ void __launch_bounds__(kBlock) kPauliBlock(Amp<T> *__restrict__ a, unsigned numTiles, PauliTile tile,
                                                       const PassDev *__restrict__ passes, int numPasses,
-                                                      const PauliExpDev *__restrict__ ops, int numOps) {
+                                                      const PauliExpDev *__restrict__ ops) {
+    constexpr int kPer = 2 * PPT;  // amplitudes a thread loads and stores per tile
     extern __shared__ unsigned char smem[];
-    __shared__ OpT<T> sOps[kMaxBlockOps];
     Amp<T> *st = reinterpret_cast<Amp<T> *>(smem);
     const int dim = 1 << tile.tb;
-    const uint64_t base = insertZeros((uint64_t)blockIdx.x, tile.fixed);
-    for (int o = threadIdx.x; o < numOps; o += kBlock) {
-        const PauliExpDev e = ops[o];
-        const bool flip = (__popcll(base & e.z) & 1) != 0;
-        OpT<T> t;
-        t.lz = e.lz;
-        t.diag = e.diag;
-        t.c = (T)e.c;
-        t.s = flip ? -(T)e.s : (T)e.s;
-        t.kre = flip ? -(T)e.kre : (T)e.kre;
-        t.kim = flip ? -(T)e.kim : (T)e.kim;
-        sOps[o] = t;
-    }
+    const int half = dim >> 1;
     // tile index t = threadIdx.x + kBlock k: the scatter of its bits splits the same way
     const int lowBits = tile.tb < 8 ? tile.tb : 8;
-    uint64_t dt = base;
+    uint64_t dt = 0;
     for (int j = 0; j < lowBits; ++j)
         if ((threadIdx.x >> j) & 1) dt |= 1ull << tile.pos[j];
-    const bool active = (int)threadIdx.x < dim;
-    for (int k = 0; active && (k << 8) < dim; ++k) {
+    uint64_t dk[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
         uint64_t g = dt;
         for (int j = 8; j < tile.tb; ++j)
             if ((k >> (j - 8)) & 1) g |= 1ull << tile.pos[j];
-        st[threadIdx.x + (k << 8)] = a[g];
+        dk[k] = g;
     }
-    __syncthreads();
-    for (int ps = 0; ps < numPasses; ++ps) {
-        const PassDev pass = passes[ps];
-        if (pass.lx == 0) {
-            // diagonal rotations: own amplitudes, no exchange
-            for (int t = threadIdx.x; t < dim; t += kBlock) {
-                Amp<T> v = st[t], dummy = v;
-                for (int o = 0; o < pass.count; ++o) rotatePair<T>(v, dummy, sOps[pass.first + o], t, t);
-                st[t] = v;
+    const bool active = (int)threadIdx.x < dim;
+    Amp<T> next[kPer];
+    unsigned w = blockIdx.x;
+    if (w < numTiles) {
+        const uint64_t nb = insertZeros((uint64_t)w, tile.fixed);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (active && (k << 8) < dim) next[k] = a[nb | dk[k]];
+    }
+    for (; w < numTiles; w += gridDim.x) {
+        const uint64_t base = insertZeros((uint64_t)w, tile.fixed);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (active && (k << 8) < dim) st[threadIdx.x + (k << 8)] = next[k];
+        __syncthreads();
+        const unsigned wn = w + gridDim.x;
+        if (wn < numTiles) {
+            const uint64_t nb = insertZeros((uint64_t)wn, tile.fixed);
+#pragma unroll
+            for (int k = 0; k < kPer; ++k)
+                if (active && (k << 8) < dim) next[k] = a[nb | dk[k]];
+        }
+        for (int ps = 0; ps < numPasses; ++ps) {
+            const PassDev pass = passes[ps];
+            // pairs (i, i ^ lx) of this thread; lx == 0: amplitudes i and i + half
+            const unsigned lx = pass.lx;
+            const unsigned low = lx & (~lx + 1u);
+            const int lb = lx ? __ffs(lx) - 1 : 0;
+            unsigned idx[PPT];
+            bool ok[PPT];
+            Amp<T> vi[PPT], vp[PPT];
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                const unsigned u = threadIdx.x + kBlock * k;
+                ok[k] = (int)u < half;
+                idx[k] = lx ? (((u >> lb) << (lb + 1)) | (u & (low - 1u))) : u;
+                if (ok[k]) {
+                    vi[k] = st[idx[k]];
+                    vp[k] = st[lx ? (idx[k] ^ lx) : (idx[k] + half)];
+                }
+            }
+            for (int o = 0; o < pass.count; ++o) {
+                const PauliExpDev e = ops[pass.first + o];
+                const bool flip = (__popcll(base & e.z) & 1) != 0;
+                const T c = (T)e.c;
+                if (e.diag) {
+                    const T s = flip ? -(T)e.s : (T)e.s;
+#pragma unroll
+                    for (int k = 0; k < PPT; ++k) {
+                        const unsigned pi = (unsigned)__popc(idx[k] & e.lz) & 1u;
+                        const unsigned pp = lx ? (pi ^ e.pxz) : ((unsigned)__popc((idx[k] + half) & e.lz) & 1u);
+                        const T fi = pi ? -s : s, fp = pp ? -s : s;
+                        const Amp<T> x = vi[k], y = vp[k];
+                        vi[k] = Amp<T>{c * x.x + fi * x.y, c * x.y - fi * x.x};
+                        vp[k] = Amp<T>{c * y.x + fp * y.y, c * y.y - fp * y.x};
+                    }
+                } else {
+                    const T kre = flip ? -(T)e.kre : (T)e.kre, kim = flip ? -(T)e.kim : (T)e.kim;
+#pragma unroll
+                    for (int k = 0; k < PPT; ++k) {
+                        // the sign on i's partner term is p's parity, and vice versa
+                        const unsigned pi = (unsigned)__popc(idx[k] & e.lz) & 1u;
+                        const T si = (pi ^ e.pxz) ? (T)-1 : (T)1;
+                        const T sp = pi ? (T)-1 : (T)1;
+                        const Amp<T> x = vi[k], y = vp[k];
+                        vi[k] = Amp<T>{c * x.x + si * (kre * y.x - kim * y.y), c * x.y + si * (kre * y.y + kim * y.x)};
+                        vp[k] = Amp<T>{c * y.x + sp * (kre * x.x - kim * x.y), c * y.y + sp * (kre * x.y + kim * x.x)};
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                if (ok[k]) {
+                    st[idx[k]] = vi[k];
+                    st[lx ? (idx[k] ^ lx) : (idx[k] + half)] = vp[k];
+                }
             }
             __syncthreads();
-            continue;
         }
-        const unsigned low = pass.lx & (~pass.lx + 1u);
-        const int lb = __ffs(pass.lx) - 1;
-        for (int w = threadIdx.x; w < (dim >> 1); w += kBlock) {
-            const unsigned i = (((unsigned)w >> lb) << (lb + 1)) | ((unsigned)w & (low - 1u));
-            const unsigned p = i ^ pass.lx;
-            Amp<T> vi = st[i], vp = st[p];
-            for (int o = 0; o < pass.count; ++o) rotatePair<T>(vi, vp, sOps[pass.first + o], i, p);
-            st[i] = vi;
-            st[p] = vp;
-        }
-        __syncthreads();
-    }
-    for (int k = 0; active && (k << 8) < dim; ++k) {
-        uint64_t g = dt;
-        for (int j = 8; j < tile.tb; ++j)
-            if ((k >> (j - 8)) & 1) g |= 1ull << tile.pos[j];
-        a[g] = st[threadIdx.x + (k << 8)];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (active && (k << 8) < dim) a[base | dk[k]] = st[threadIdx.x + (k << 8)];
+        __syncthreads();  // the next tile overwrites the LDS copy
     }
 }
+
+constexpr int kMaxBlockOps = 1 << 16;  // rotations per block (bounds the host's records only)
 
 struct Block {
     PauliTile tile;
@@ -220,6 +246,12 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps) {
         }
         blk.numPasses = (int)passes.size() - blk.firstPass;
         blk.numOps = (int)ops.size() - blk.firstOp;
+        // partner signs: parity(lx & lz) with each pass's final exchange mask
+        for (int ps = blk.firstPass; ps < (int)passes.size(); ++ps)
+            for (int o = 0; o < passes[ps].count; ++o) {
+                PauliExpDev &d = ops[blk.firstOp + passes[ps].first + o];
+                d.pxz = (unsigned)__builtin_popcount(passes[ps].lx & d.lz) & 1u;
+            }
         if (blk.numPasses > 0) blocks.push_back(blk);
         i = j;
     }
@@ -240,11 +272,19 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps) {
                 continue;
             }
             const unsigned tiles = (unsigned)(q->numAmps >> b.tile.tb);
-            QMX_DISPATCH(q->prec, T, {
-                hipLaunchKernelGGL(kPauliBlock<T>, dim3(tiles), dim3(kBlock), (size_t)sizeof(Amp<T>) << b.tile.tb,
-                                   hstream(), (Amp<T> *)q->amps, b.tile, dPasses + b.firstPass, b.numPasses,
-                                   dOps + b.firstOp, b.numOps);
-            });
+            // one workgroup per tile measured best (a persistent grid of 3 per CU
+            // that prefetches its next tile was no faster: the passes are
+            // VALU-bound, not waiting on HBM)
+            const unsigned grid = tiles;
+            // pairs per thread: 2^tb / 2 / kBlock at the full tile size (8 FP32, 4 FP64)
+            if (q->prec == FP64)
+                hipLaunchKernelGGL((kPauliBlock<double, 4>), dim3(grid), dim3(kBlock),
+                                   (size_t)sizeof(Amp<double>) << b.tile.tb, hstream(), (Amp<double> *)q->amps, tiles,
+                                   b.tile, dPasses + b.firstPass, b.numPasses, dOps + b.firstOp);
+            else
+                hipLaunchKernelGGL((kPauliBlock<float, 8>), dim3(grid), dim3(kBlock),
+                                   (size_t)sizeof(Amp<float>) << b.tile.tb, hstream(), (Amp<float> *)q->amps, tiles,
+                                   b.tile, dPasses + b.firstPass, b.numPasses, dOps + b.firstOp);
         }
         checkLaunch("applyPauliSequence");
     }
