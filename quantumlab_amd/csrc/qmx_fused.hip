@@ -8,15 +8,19 @@
 // each the sum of 16 v_mfma_f32_16x16x4f32 steps over K = 64.
 //
 // Operand placement (16x16x4 f32 MFMA, wave64):
-//   A (16x4): lane l holds A[l%16][l/16]  -> M fragments, 64 floats per lane,
-//             loaded once per wave and kept in VGPRs for all its groups;
+//   A (16x4): lane l holds A[l%16][l/16]  -> M fragments, staged once per
+//             workgroup in LDS as [row tile][row][k lane][step] so a lane
+//             reads its 16 values of a row tile as four 16 B vectors;
 //   B (4x16): lane l holds B[l/16][l%16]  -> lane l loads amplitude
 //             (group l%16, combination 4s + l/16) for s = 0..7: its .x feeds
 //             K-slice s (real rows), its .y K-slice s+8 (imaginary rows);
 //   D (16x16): lane l holds D[4(l/16)+v][l%16], v = 0..3.
-// So amplitudes go from HBM straight into MFMA operands and back, with no LDS
-// staging; 16 lanes of a wave read 16 consecutive groups, i.e. consecutive
-// addresses whenever the lowest qubit is not a target.
+// Amplitudes go from HBM straight into MFMA operands and back. The block is
+// indexed by ascending target (combination bit j = j-th lowest target), so a
+// load's 4 K lanes run over the two lowest targets and its 16 columns over
+// consecutive groups: whole cache lines whenever the low qubits are targets or
+// groups. Each wave takes 4 consecutive 16-group chunks and loads the next
+// chunk's amplitudes before multiplying the current one.
 //
 // Cost per amplitude: 256 FP32 MFMA flops for 16 B of HBM traffic, which on
 // MI355X (157.3 TFLOP/s FP32 matrix, 8 TB/s HBM) keeps the block near the
@@ -37,39 +41,59 @@ struct Fused5Spec {
     uint64_t off[32];        // offset of combination c: sum_j bit_j(c) << targs[j]
 };
 
-constexpr int kChunksPerWave = 8;  // 16-group chunks handled by one wave
+constexpr int kChunksPerWave = 4;  // 16-group chunks handled by one wave
 
 __global__ void __launch_bounds__(kBlock) kFused5(Amp<float> *__restrict__ a, long long numChunks,
                                                   Fused5Spec spec, const float *__restrict__ M) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
-    // A fragments: Af[rt][s] = M[16 rt + col][4 s + kq]
-    float Af[4][16];
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-        for (int s = 0; s < 16; ++s) Af[rt][s] = M[(16 * rt + col) * 64 + 4 * s + kq];
+    // A fragments staged in LDS as [rt][col][kq][s] so a lane's 16 values of a
+    // row tile are contiguous
+    __shared__ float sM[4 * 16 * 4 * 16];
+    for (int e = threadIdx.x; e < 4096; e += kBlock) {
+        const int rt = e >> 10, cc = (e >> 6) & 15, kk = (e >> 4) & 3, ss = e & 15;
+        sM[e] = M[(16 * rt + cc) * 64 + 4 * ss + kk];
+    }
+    __syncthreads();
 
     const long long firstChunk = (xcdBlock() * (kBlock / 64) + wave) * kChunksPerWave;
+    // the next chunk's amplitudes are loaded while the current chunk multiplies
+    Amp<float> x[8], xn[8];
+    uint64_t base = 0, nbase = 0;
+    if (firstChunk < numChunks) {
+        nbase = insertZeros((uint64_t)firstChunk * 16 + col, spec.targetsSorted);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) xn[s] = ldA<float, false>(a + (nbase + spec.off[4 * s + kq]));
+    }
     for (int it = 0; it < kChunksPerWave; ++it) {
         const long long chunk = firstChunk + it;
         if (chunk >= numChunks) break;
-        const uint64_t g = (uint64_t)chunk * 16 + col;
-        const uint64_t base = insertZeros(g, spec.targetsSorted);
-        Amp<float> x[8];
+        base = nbase;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) x[s] = ldA<float, true>(a + (base + spec.off[4 * s + kq]));
+        for (int s = 0; s < 8; ++s) x[s] = xn[s];
+        if (it + 1 < kChunksPerWave && chunk + 1 < numChunks) {
+            nbase = insertZeros((uint64_t)(chunk + 1) * 16 + col, spec.targetsSorted);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) xn[s] = ldA<float, false>(a + (nbase + spec.off[4 * s + kq]));
+        }
         f4 acc[4];
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
+            float Af[1][16];
+            const f4 *src = reinterpret_cast<const f4 *>(sM + (((rt * 16 + col) * 4 + kq) << 4));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const f4 v = src[u];
+                Af[0][4 * u] = v[0]; Af[0][4 * u + 1] = v[1]; Af[0][4 * u + 2] = v[2]; Af[0][4 * u + 3] = v[3];
+            }
             acc[rt] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s = 0; s < 8; ++s)
-                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[rt][s], x[s].x, acc[rt], 0, 0, 0);
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[0][s], x[s].x, acc[rt], 0, 0, 0);
 #pragma unroll
             for (int s = 0; s < 8; ++s)
-                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[rt][s + 8], x[s].y, acc[rt], 0, 0, 0);
+                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[0][s + 8], x[s].y, acc[rt], 0, 0, 0);
         }
         // lane holds output rows 4 kq + v of each tile: combination c = 16 h + 4 kq + v,
         // real part from tile h, imaginary part from tile h + 2
@@ -78,7 +102,7 @@ __global__ void __launch_bounds__(kBlock) kFused5(Amp<float> *__restrict__ a, lo
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int c = 16 * h + 4 * kq + v;
-                stA<float, true>(a + (base + spec.off[c]), Amp<float>{acc[h][v], acc[h + 2][v]});
+                stA<float, false>(a + (base + spec.off[c]), Amp<float>{acc[h][v], acc[h + 2][v]});
             }
     }
 }
@@ -94,21 +118,29 @@ struct Fused5Prepared {
 // significant bit of u's index, u row-major 32x32).
 void *fused5Prepare(const int *targs, const Cplx *u) {
     auto *p = new Fused5Prepared;
-    uint64_t tmask = 0;
-    for (int j = 0; j < 5; ++j) tmask |= 1ull << targs[j];
-    p->spec.targetsSorted.n = 0;
-    for (int b = 0; b < 64; ++b)
-        if ((tmask >> b) & 1ull) p->spec.targetsSorted.pos[p->spec.targetsSorted.n++] = b;
+    // index the block by ascending target: bit j of a combination is the j-th
+    // lowest target, so a wave's 4 combinations (MFMA K lanes) run over the
+    // lowest targets and its loads cover whole cache lines
+    int order[5] = {0, 1, 2, 3, 4};
+    std::sort(order, order + 5, [&](int x, int y) { return targs[x] < targs[y]; });
+    p->spec.targetsSorted.n = 5;
+    for (int j = 0; j < 5; ++j) p->spec.targetsSorted.pos[j] = targs[order[j]];
+    int toOld[32];
     for (int c = 0; c < 32; ++c) {
         uint64_t o = 0;
+        int old = 0;
         for (int j = 0; j < 5; ++j)
-            if ((c >> j) & 1) o |= 1ull << targs[j];
+            if ((c >> j) & 1) {
+                o |= 1ull << targs[order[j]];
+                old |= 1 << order[j];
+            }
         p->spec.off[c] = o;
+        toOld[c] = old;
     }
     std::vector<float> hm(64 * 64);
     for (int r = 0; r < 32; ++r)
         for (int c = 0; c < 32; ++c) {
-            const Cplx z = u[r * 32 + c];
+            const Cplx z = u[toOld[r] * 32 + toOld[c]];
             hm[r * 64 + c] = (float)z.re;
             hm[r * 64 + 32 + c] = (float)-z.im;
             hm[(32 + r) * 64 + c] = (float)z.im;
