@@ -51,6 +51,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GB/s+gates/s at 32q FP64; swap-tests/s batched; Trotter solve time at F≥1-1e-4"
 HBM_PEAK_GBS = 8000.0
+FP32_MATRIX_PEAK_TFLOPS = 157.3
 PERIOD = 5
 
 
@@ -182,6 +183,10 @@ def run_density(qe, n, layers, seed):
     qe.destroy(pure)
     a, b = outs["fp64"], outs["fp32"]
     res.update(outs)
+    res["roofline"] = {"bound": "hbm", "kernel": "kChannel1 / kMatrix2 / kNot on the 2n-qubit vector (FP64)",
+                       "achieved": a["gb_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(a["gb_per_s"] / HBM_PEAK_GBS, 4), "traffic": None,
+                       "note": "algorithmic bytes of every gate and channel over the leg's wall time"}
     res["fp32_vs_fp64"] = {"purity_error": abs(a["purity"] - b["purity"]),
                            "fidelity_error": abs(a["fidelity_vs_noiseless"] - b["fidelity_vs_noiseless"]),
                            "max_prob_error": max(abs(x - y) for x, y in zip(a["prob0"], b["prob0"])),
@@ -252,7 +257,8 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
             qe.qmxSync()
             sd[order] = (time.perf_counter() - t0, reps.value, ok)
         qe.destroy(q)
-        rows.append((inst, o.value, r.value, t_learned, infid, sd))
+        sweeps = qe.lib.qmxTrotterSweepsPerRep(hb[0], o.value, 0, QMX_PREC_FP32) * r.value
+        rows.append((inst, o.value, r.value, t_learned, infid, sd, sweeps))
         log(f"trotter {idx}: {inst.model} {inst.terms} terms t={inst.time:.2f}: learned order {o.value} x{r.value} "
             f"{t_learned * 1e3:.0f} ms (1-F {infid:.1e}); step doubling o2 x{sd[2][1]} {sd[2][0] * 1e3:.0f} ms, "
             f"o4 x{sd[4][1]} {sd[4][0] * 1e3:.0f} ms")
@@ -262,6 +268,8 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
     t4 = np.array([x[5][4][0] for x in rows])
     inf = np.array([x[4] for x in rows])
     exps = sum(H.stages(x[1]) * x[2] * x[0].terms for x in rows)
+    sweep_bytes = sum(x[6] for x in rows) * 2.0 * (1 << n) * 8
+    gbs = sweep_bytes / tl.sum() / 1e9
     return {"metric": "Trotter solve time at F>=1-1e-4", "unit": "ms", "higher_is_better": False,
             "workload": f"{count} seeded Hamiltonians (TFIM/Heisenberg chains + 32 random 2-4-local strings) on "
                         f"{n} qubits, random product states, t ~ U(0.5, 4); FP32 registers",
@@ -275,6 +283,12 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
             "success_rate": float(np.mean(inf <= eps)), "max_infidelity_vs_4x_finer_fp64": float(inf.max()),
             "step_doubling_met_eps": bool(all(x[5][2][2] and x[5][4][2] for x in rows)),
             "pauli_exponentials_per_s_learned": round(exps / tl.sum(), 1),
+            "roofline": {"bound": "hbm", "kernel": "kPauliBlock<float> (fused Pauli-rotation sweeps)",
+                         "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                         "sweeps": int(sum(x[6] for x in rows)), "bytes_per_sweep": 2.0 * (1 << n) * 8,
+                         "note": "read + write of the FP32 register per fused block, over the learned solves' wall "
+                                 "time; the in-tile rotation arithmetic (VALU) is what holds it below HBM"},
             "choices": [[x[1], x[2]] for x in rows],
             "step_doubling_reps": [[x[5][2][1], x[5][4][1]] for x in rows]}, rows
 
@@ -363,6 +377,10 @@ def run_swap_tests(qe, points, centroids, iterations, warmup, seed):
     ev_ms = qe.qmxTimerEndMs()
     timed_same = bool(np.array_equal(plan.assignments(), final64[0]))
     timed_err = float(np.max(np.abs(2 * plan.p0() - 2 * final64[1])))
+    # the swap-test tile kernel alone: MFMA blocks per launch / device time
+    blocks = int(qe.lib.qmxSwapPlanMfmaBlocks(plan.plan))
+    tile_ms = float(qe.lib.qmxSwapPlanTimeTiles(plan.plan, 5))
+    tflops = blocks * 4096 / (tile_ms / 1e3) / 1e12
     evals = K.swap_test_amplitude_evaluations(data)
     res = {"swap_tests_per_s": round(tests * iterations / wall, 1), "unit": "swap-tests/s",
            "workload": f"{points} points x {centroids} centroids, q in 4..10 (registers 2q+1 = 9..21 qubits), "
@@ -373,7 +391,14 @@ def run_swap_tests(qe, points, centroids, iterations, warmup, seed):
            "assignments_identical_to_fp64_every_iteration": same_assign,
            "max_distance_error_vs_exact_fp64": dist_err, "max_distance_error_fp64_plan": dist_err64,
            "timed_run_assignments_identical_to_fp64": timed_same, "timed_run_max_distance_diff_vs_fp64": timed_err,
-           "autotune_timings": timings, "rows_per_tile": rows, "tiles": plan.num_tiles()}
+           "autotune_timings": timings, "rows_per_tile": rows, "tiles": plan.num_tiles(),
+           "roofline": {"bound": "mfma", "kernel": "kSwapTilesMfma (32x32x2 FP32 MFMA blocks of the amplitude matrix)",
+                        "achieved": round(tflops, 2), "peak": FP32_MATRIX_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tflops / FP32_MATRIX_PEAK_TFLOPS, 4), "traffic": None,
+                        "avg_launch_ms": round(tile_ms, 4), "mfma_blocks_per_launch": blocks,
+                        "flops_per_block": 4096,
+                        "note": "registers are L2-resident (19 MB of FP32 point registers), so HBM does not bound it; "
+                                "each block also costs 16 FP32 FMAs per lane for the squares"}}
     plan.close()
     p64.close()
     return res, data
@@ -544,6 +569,13 @@ def main():
                                 "gates_per_s": round(len(timed) / wall32, 2),
                                 "fidelity_error": float(1 - fid), "max_abs_amp_error": float(maxabs),
                                 "max_prob_error": float(p_err), "norm2_before_renorm": float(norm2)}
+        fgbs = blocks * 2.0 * N * 8 / wall32 / 1e9
+        result["fused_fp32"]["roofline"] = {
+            "bound": "hbm", "kernel": "kFused5 (5-qubit blocks, v_mfma_f32_16x16x4f32)", "achieved": round(fgbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(fgbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_block": 2.0 * N * 8,
+            "mfma_tflops": round(blocks * N * 256 / wall32 / 1e12, 2),
+            "note": "every block reads and writes the FP32 register once; 256 MFMA flops per amplitude"}
         log(f"fused fp32: {wall / wall32:.2f}x vs fp64 per-gate, 1-F={1 - fid:.3e}, max|da|={maxabs:.3e}")
         qe.destroy(q32)
     if q64 is not None:

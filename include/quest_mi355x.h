@@ -95,6 +95,10 @@ void qmxSwapPlanDestroy(qmxSwapPlan *plan);
 int qmxSwapPlanAutotune(qmxSwapPlan *plan);
 int qmxSwapPlanRowsPerTile(const qmxSwapPlan *plan, int qubits);
 int qmxSwapPlanNumTiles(const qmxSwapPlan *plan);
+/* 32x32 amplitude blocks an iteration forms on the matrix cores (FP32 plans; 0 for FP64) */
+long long qmxSwapPlanMfmaBlocks(const qmxSwapPlan *plan);
+/* device time of one swap-test launch (ms), averaged over `launches` (overwrites the partials) */
+double qmxSwapPlanTimeTiles(qmxSwapPlan *plan, int launches);
 void qmxSwapPlanRun(qmxSwapPlan *plan, int iterations, int useGraph);
 /* P(ancilla = 0) of the last iteration's tests, numPoints x numCentroids */
 void qmxSwapPlanP0(qmxSwapPlan *plan, double *out);
@@ -147,6 +151,10 @@ typedef struct qmxPlanner qmxPlanner;
 int qmxTrotterFeatures(PauliHamil hamil, qreal time, float *out16);
 /* Pauli-exponential passes per repetition: 1 (order 1), 2, 10, 50, ... */
 int qmxTrotterStages(int order);
+/* state sweeps one repetition of applyTrotterCircuit(order) makes on a
+ * register of this size and precision (fused rotation blocks plus rotations
+ * too wide for a tile) */
+int qmxTrotterSweepsPerRep(PauliHamil hamil, int order, int isDensityMatrix, int precision);
 qmxPlanner *qmxPlannerCreate(unsigned seed);  /* He-uniform weights, zero biases */
 void qmxPlannerDestroy(qmxPlanner *p);
 int qmxPlannerNumParams(void);                 /* 5378 */

@@ -663,6 +663,39 @@ int qmxSwapPlanRowsPerTile(const qmxSwapPlan *plan, int q) {
 
 int qmxSwapPlanNumTiles(const qmxSwapPlan *plan) { return plan->impl.numTiles; }
 
+// 32 x 32 amplitude blocks one iteration forms on the matrix cores (FP32
+// plans: blocks on or above the diagonal of every test, padded to a block for
+// registers under 32 amplitudes); 0 for FP64 plans
+long long qmxSwapPlanMfmaBlocks(const qmxSwapPlan *plan) {
+    using namespace qmx;
+    const SwapPlanImpl *s = &plan->impl;
+    if (s->prec != FP32) return 0;
+    long long blocks = 0;
+    for (int p = 0; p < s->P; ++p) {
+        const int D = 1 << s->hQ[p];
+        const int rows = std::min(std::min(D, 32 * kMaxRowBlocks), s->rowsPerTile[s->hQ[p]]);
+        const int nyb = (D + 31) / 32;
+        for (int x0 = 0; x0 < D; x0 += rows) {
+            const int xb0 = x0 / 32, nb = (std::min(rows, D - x0) + 31) / 32;
+            for (int k = 0; k < nb; ++k) blocks += nyb - (xb0 + k);
+        }
+    }
+    return blocks * s->C;
+}
+
+// device time of one launch of the swap-test tiles (ms, HIP events on the
+// library stream), averaged over `launches`
+double qmxSwapPlanTimeTiles(qmxSwapPlan *plan, int launches) {
+    using namespace qmx;
+    SwapPlanImpl *s = &plan->impl;
+    if (launches < 1) return 0.0;
+    deviceSync();
+    timerBegin();
+    for (int i = 0; i < launches; ++i) QMX_DISPATCH(s->prec, T, { launchSwapTiles<T>(s); });
+    checkLaunch("qmxSwapPlanTimeTiles");
+    return timerEndMs() / launches;
+}
+
 // run `iterations` k-means iterations; useGraph != 0 replays one captured
 // iteration graph (captured on first use)
 void qmxSwapPlanRun(qmxSwapPlan *plan, int iterations, int useGraph) {

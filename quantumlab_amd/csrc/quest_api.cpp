@@ -1214,6 +1214,19 @@ void applyPauliHamil(Qureg inQureg, PauliHamil hamil, Qureg outQureg) {
     API_CATCH_VOID
 }
 
+int qmxTrotterSweepsPerRep(PauliHamil hamil, int order, int isDensityMatrix, int precision) {
+    if (hamil.numQubits < 1 || hamil.numSumTerms < 1 || (order != 1 && (order < 2 || order % 2))) return 0;
+    Qureg shape;
+    std::memset(&shape, 0, sizeof shape);
+    shape.isDensityMatrix = isDensityMatrix != 0;
+    shape.numQubitsRepresented = hamil.numQubits;
+    std::vector<qmx::PauliExp> seq;
+    symmetrizedTrotter(seq, shape, hamil, 1.0, order);
+    const int nq = isDensityMatrix ? 2 * hamil.numQubits : hamil.numQubits;
+    return qmx::pauliSequenceBlocks(nq, precision == QMX_PREC_FP32 ? qmx::FP32 : qmx::FP64, seq.data(),
+                                    (int)seq.size());
+}
+
 void applyTrotterCircuit(Qureg qureg, PauliHamil hamil, qreal time, int order, int reps) {
     API_TRY
     qv::trotterParams(order, reps, __func__);

@@ -261,6 +261,8 @@ _EXT_SIGS = {
     "qmxSwapPlanAutotune": (C.c_int, [C.c_void_p]),
     "qmxSwapPlanRowsPerTile": (C.c_int, [C.c_void_p, C.c_int]),
     "qmxSwapPlanNumTiles": (C.c_int, [C.c_void_p]),
+    "qmxSwapPlanMfmaBlocks": (C.c_longlong, [C.c_void_p]),
+    "qmxSwapPlanTimeTiles": (C.c_double, [C.c_void_p, C.c_int]),
     "qmxSwapPlanRun": (None, [C.c_void_p, C.c_int, C.c_int]),
     "qmxSwapPlanP0": (None, [C.c_void_p, C.POINTER(C.c_double)]),
     "qmxSwapPlanAssignments": (None, [C.c_void_p, C.POINTER(C.c_int)]),
@@ -278,6 +280,7 @@ _EXT_SIGS = {
                                         C.POINTER(C.c_int)]),
     "qmxTrotterFeatures": (C.c_int, [PauliHamil, qreal, C.POINTER(C.c_float)]),
     "qmxTrotterStages": (C.c_int, [C.c_int]),
+    "qmxTrotterSweepsPerRep": (C.c_int, [PauliHamil, C.c_int, C.c_int, C.c_int]),
     "qmxPlannerCreate": (C.c_void_p, [C.c_uint]),
     "qmxPlannerDestroy": (None, [C.c_void_p]),
     "qmxPlannerNumParams": (C.c_int, []),
