@@ -49,6 +49,12 @@ double qmxTimerEndMs(void);
 void qmxTimerSlotBegin(int slot);
 void qmxTimerSlotEnd(int slot);
 double qmxTimerSlotMs(int slot);
+/* Density registers: operations on at most two qubits (gates, controlled
+ * gates, channels, Kraus maps) compose into one superoperator that is swept
+ * over the register once, when the next operation needs other qubits, another
+ * register or the stream. Cumulative counts since load: sweeps made and
+ * operations absorbed into them. */
+void qmxDensityFusionStats(long long *sweeps, long long *operations);
 /* max |a_i - b_i| over amplitudes (any precisions, same size) */
 qreal qmxMaxAbsDiff(Qureg a, Qureg b);
 /* FP64 norm of the register; if rescale != 0 the register is renormalised */

@@ -780,6 +780,47 @@ void mixKrausMap(Qureg q, int t, ComplexMatrix2 *ops, int numOps) {
     int targs[2] = {t, t + shiftOf(q)};
     sv_matrixN(q, 0, targs, 2, ur, ui);
 }
+/* two-qubit Kraus superoperator on (t1, t2, t1+n, t2+n) (QuEST_common.c:679-700,
+ * QuEST.c:1333-1342); superoperator index = row index + 4 * column index */
+void mixTwoQubitKrausMap(Qureg q, int t1, int t2, ComplexMatrix4 *ops, int numOps) {
+    static qreal ur[256], ui[256];
+    memset(ur, 0, sizeof ur);
+    memset(ui, 0, sizeof ui);
+    for (int n = 0; n < numOps; n++)
+        for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) for (int k = 0; k < 4; k++) for (int l = 0; l < 4; l++) {
+            int r = i * 4 + k, c = j * 4 + l;
+            ur[r * 16 + c] += ops[n].real[i][j] * ops[n].real[k][l] + ops[n].imag[i][j] * ops[n].imag[k][l];
+            ui[r * 16 + c] += ops[n].real[i][j] * ops[n].imag[k][l] - ops[n].imag[i][j] * ops[n].real[k][l];
+        }
+    int targs[4] = {t1, t2, t1 + shiftOf(q), t2 + shiftOf(q)};
+    sv_matrixN(q, 0, targs, 4, ur, ui);
+}
+/* rho -> (1-p) rho + p/15 sum of the 15 non-identity Pauli pairs (QuEST.c:1300-1310);
+ * restated as its 16 Kraus operators sqrt(1-p) I, sqrt(p/15) s_a (x) s_b */
+void mixTwoQubitDepolarising(Qureg q, int t1, int t2, qreal prob) {
+    static const qreal pr[4][2][2] = {{{1, 0}, {0, 1}}, {{0, 1}, {1, 0}}, {{0, 0}, {0, 0}}, {{1, 0}, {0, -1}}};
+    static const qreal pi[4][2][2] = {{{0, 0}, {0, 0}}, {{0, 0}, {0, 0}}, {{0, -1}, {1, 0}}, {{0, 0}, {0, 0}}};
+    ComplexMatrix4 ops[16];
+    memset(ops, 0, sizeof ops);
+    for (int a = 0; a < 4; a++)      /* Pauli on t1 (index bit 0) */
+        for (int b = 0; b < 4; b++) { /* Pauli on t2 (index bit 1) */
+            qreal f = (a == 0 && b == 0) ? sqrt(1 - prob) : sqrt(prob / 15);
+            ComplexMatrix4 *m = &ops[b * 4 + a];
+            for (int r = 0; r < 4; r++) for (int c = 0; c < 4; c++) {
+                qreal ar = pr[a][r & 1][c & 1], ai = pi[a][r & 1][c & 1];
+                qreal br = pr[b][r >> 1][c >> 1], bi = pi[b][r >> 1][c >> 1];
+                m->real[r][c] = f * (ar * br - ai * bi);
+                m->imag[r][c] = f * (ar * bi + ai * br);
+            }
+        }
+    mixTwoQubitKrausMap(q, t1, t2, ops, 16);
+}
+/* QuEST.c:955-963: element (row, col) of the flat column-major matrix */
+Complex getDensityAmp(Qureg q, ll row, ll col) {
+    ll i = row + (col << q.numQubitsRepresented);
+    Complex z = {q.stateVec.real[i], q.stateVec.imag[i]};
+    return z;
+}
 void mixPauli(Qureg q, int t, qreal px, qreal py, qreal pz) {
     ComplexMatrix2 ops[4];
     memset(ops, 0, sizeof ops);

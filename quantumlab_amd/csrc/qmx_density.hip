@@ -1,11 +1,19 @@
-// Measurement collapse and density-matrix channels.
+// Measurement collapse and density-matrix superoperators.
 //
 // A density matrix of n qubits is a 2n-qubit vector; element (r, c) sits at
-// r + c*2^n, so "row bit q" is bit q and "column bit q" is bit q+n. A
-// one-qubit channel on q touches quads {00, 01, 10, 11} of (row bit, column
-// bit). Each channel is a single pass over its quads; the reference GPU
-// backend ran a separate dephasing pass before depolarising / damping
-// (QuEST_gpu.cu:3027-3078).
+// r + c*2^n, so "row bit q" is bit q and "column bit q" is bit q+n. A unitary
+// U on the register maps vec(rho) by conj(U) (x) U; a one-qubit channel maps
+// the quad {00, 01, 10, 11} of (row bit, column bit). The reference applies a
+// gate as two sweeps (rows, then conjugated columns; QuEST.c:410-421)
+// and each channel as its own sweep, a dephasing sweep first for depolarising
+// and damping (QuEST_gpu.cu:3027-3078). Here every operation on at most two
+// qubits composes on the host into one 4^k x 4^k superoperator (k <= 2), and
+// the register is swept once per composed run: a brick-circuit gate with its
+// three channels per touched qubit is one sweep instead of five to eight.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "qmx_device.h"
 
 namespace qmx {
@@ -52,34 +60,40 @@ __global__ void __launch_bounds__(kBlock) kDensCollapse(Amp<T> *__restrict__ a, 
     }
 }
 
-// MODE 0 dephase, 1 depolarise, 2 damp (QuEST_cpu.c:61-95, 138-235)
-template <typename T, int MODE>
-__global__ void __launch_bounds__(kBlock) kChannel1(Amp<T> *__restrict__ a, long long work, BitList fixed,
-                                                    uint64_t rowM, uint64_t colM, double p1, double p2) {
-    long long w = (long long)blockIdx.x * kBlock * kPer + threadIdx.x;
+// Dense superoperator on the 4^K entries of rho that share every bit outside
+// the K targets' row and column bits: one thread per group, entries held in
+// registers, the matrix read through scalar loads (uniform across the wave).
+struct SuperSpec {
+    int fix[4];          // ascending row/column bit positions of the targets
+    uint64_t off[16];    // address offset of group entry e
+};
+
+template <typename T, int K, bool NT>
+__global__ void __launch_bounds__(kBlock) kSuperOp(Amp<T> *__restrict__ a, long long groups, SuperSpec sp,
+                                                   const Amp<T> *__restrict__ S) {
+    constexpr int E = 1 << (2 * K);
+    const long long g = xcdBlock() * kBlock + threadIdx.x;
+    if (g >= groups) return;
+    uint64_t base = (uint64_t)g;
 #pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        long long wj = w + (long long)j * kBlock;
-        if (wj >= work) continue;
-        const uint64_t b00 = insertZeros((uint64_t)wj, fixed);
-        const uint64_t b01 = b00 | rowM, b10 = b00 | colM, b11 = b00 | rowM | colM;
-        Amp<T> x01 = a[b01], x10 = a[b10];
-        const double off = p1;  // factor on the off-diagonal pair
-        a[b01] = Amp<T>{(T)(off * x01.x), (T)(off * x01.y)};
-        a[b10] = Amp<T>{(T)(off * x10.x), (T)(off * x10.y)};
-        if (MODE == 1) {
-            // depolarise: d = p2, retain = 1-d; diag -> retain*x + d*(x00+x11)/2
-            Amp<T> x00 = a[b00], x11 = a[b11];
-            double ar = ((double)x00.x + x11.x) / 2, ai = ((double)x00.y + x11.y) / 2;
-            double keep = 1 - p2;
-            a[b00] = Amp<T>{(T)(keep * x00.x + p2 * ar), (T)(keep * x00.y + p2 * ai)};
-            a[b11] = Amp<T>{(T)(keep * x11.x + p2 * ar), (T)(keep * x11.y + p2 * ai)};
-        } else if (MODE == 2) {
-            // damp: g = p2; |0><0| += g |1><1|; |1><1| *= (1-g)
-            Amp<T> x00 = a[b00], x11 = a[b11];
-            a[b00] = Amp<T>{(T)(x00.x + p2 * x11.x), (T)(x00.y + p2 * x11.y)};
-            a[b11] = Amp<T>{(T)((1 - p2) * x11.x), (T)((1 - p2) * x11.y)};
+    for (int j = 0; j < 2 * K; ++j) base = insertZero(base, sp.fix[j]);
+    T xr[E], xi[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        Amp<T> v = ldA<T, NT>(a + (base | sp.off[e]));
+        xr[e] = v.x;
+        xi[e] = v.y;
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        T yr = 0, yi = 0;
+#pragma unroll
+        for (int c = 0; c < E; ++c) {
+            const Amp<T> m = S[r * E + c];
+            yr = fma(m.x, xr[c], fma(-m.y, xi[c], yr));
+            yi = fma(m.x, xi[c], fma(m.y, xr[c], yi));
         }
+        stA<T, NT>(a + (base | sp.off[r]), Amp<T>{yr, yi});
     }
 }
 
@@ -102,26 +116,6 @@ __global__ void __launch_bounds__(kBlock) kDephase2(Amp<T> *__restrict__ a, long
     }
 }
 
-BitList bits2(int p, int q) {
-    BitList b;
-    b.n = 2;
-    b.pos[0] = p < q ? p : q;
-    b.pos[1] = p < q ? q : p;
-    return b;
-}
-
-template <int MODE>
-void channel1(QuregDev *q, int qubit, double p1, double p2) {
-    const int shift = q->numQubitsRep;
-    const long long work = q->numAmps >> 2;
-    BitList fixed = bits2(qubit, qubit + shift);
-    QMX_DISPATCH(q->prec, T, {
-        hipLaunchKernelGGL((kChannel1<T, MODE>), dim3(gridFor(work, kPer)), dim3(kBlock), 0, hstream(),
-                           (Amp<T> *)q->amps, work, fixed, 1ull << qubit, 1ull << (qubit + shift), p1, p2);
-    });
-    checkLaunch("density channel");
-}
-
 }  // namespace
 
 void collapse(QuregDev *q, int qubit, int outcome, double prob) {
@@ -142,22 +136,237 @@ void densCollapse(QuregDev *q, int qubit, int outcome, double prob) {
     checkLaunch("densCollapse");
 }
 
-void densDephase(QuregDev *q, int qubit, double retain) { channel1<0>(q, qubit, retain, 0.0); }
-
-void densDepolarise(QuregDev *q, int qubit, double depolLevel) {
-    channel1<1>(q, qubit, 1.0 - depolLevel, depolLevel);
-}
-
-void densDamp(QuregDev *q, int qubit, double damping) {
-    channel1<2>(q, qubit, sqrt(1.0 - damping), damping);
-}
-
 void densTwoQubitDephase(QuregDev *q, int q1, int q2, double retain) {
     QMX_DISPATCH(q->prec, T, {
         hipLaunchKernelGGL(kDephase2<T>, dim3(gridFor(q->numAmps, kPer)), dim3(kBlock), 0, hstream(),
                            (Amp<T> *)q->amps, q->numAmps, q1, q2, q->numQubitsRep, retain);
     });
     checkLaunch("densTwoQubitDephase");
+}
+
+// ---- pending superoperator -------------------------------------------------
+namespace {
+
+constexpr int kPendMax = 2;                // qubits one pending superoperator spans
+constexpr int kMaxEntries = 256;           // (4^2)^2
+constexpr int kRing = 256;                 // matrix slots in flight
+constexpr size_t kSlotBytes = kMaxEntries * sizeof(Amp<double>);
+
+struct Pending {
+    QuregDev *q = nullptr;
+    int k = 0;
+    int t[kPendMax] = {0, 0};
+    Cplx S[kMaxEntries];
+};
+Pending g_pend;
+long long g_passes = 0, g_absorbed = 0;
+
+// pinned staging ring for the matrices: slot i is rewritten only after the copy
+// that last read it has completed; copies and launches share the stream, so
+// the device slot is reused only after the kernel that read it has finished
+struct MatrixRing {
+    char *dev = nullptr, *host = nullptr;
+    hipEvent_t ev[kRing];
+    bool used[kRing] = {};
+    int next = 0;
+};
+MatrixRing g_ring;
+
+template <typename T>
+const Amp<T> *stageMatrix(const Cplx *S, int entries, hipStream_t st) {
+    if (!g_ring.dev) {
+        g_ring.dev = static_cast<char *>(devAlloc(kRing * kSlotBytes));
+        QMX_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&g_ring.host), kRing * kSlotBytes,
+                                    hipHostMallocDefault));
+        for (int i = 0; i < kRing; ++i)
+            QMX_HIP_CHECK(hipEventCreateWithFlags(&g_ring.ev[i], hipEventDisableTiming));
+    }
+    const int slot = g_ring.next;
+    g_ring.next = (slot + 1) % kRing;
+    if (g_ring.used[slot]) QMX_HIP_CHECK(hipEventSynchronize(g_ring.ev[slot]));
+    Amp<T> *h = reinterpret_cast<Amp<T> *>(g_ring.host + slot * kSlotBytes);
+    for (int e = 0; e < entries; ++e) h[e] = Amp<T>{(T)S[e].re, (T)S[e].im};
+    char *d = g_ring.dev + slot * kSlotBytes;
+    QMX_HIP_CHECK(hipMemcpyAsync(d, h, entries * sizeof(Amp<T>), hipMemcpyHostToDevice, st));
+    QMX_HIP_CHECK(hipEventRecord(g_ring.ev[slot], st));
+    g_ring.used[slot] = true;
+    return reinterpret_cast<const Amp<T> *>(d);
+}
+
+template <typename T, int K>
+void launchSuperOpT(QuregDev *q, long long groups, const SuperSpec &sp, bool nt, const Cplx *S,
+                    hipStream_t st) {
+    const Amp<T> *m = stageMatrix<T>(S, 1 << (4 * K), st);
+    const unsigned grid = gridFor(groups, 1);
+    if (nt)
+        hipLaunchKernelGGL((kSuperOp<T, K, true>), dim3(grid), dim3(kBlock), 0, st, (Amp<T> *)q->amps, groups,
+                           sp, m);
+    else
+        hipLaunchKernelGGL((kSuperOp<T, K, false>), dim3(grid), dim3(kBlock), 0, st, (Amp<T> *)q->amps,
+                           groups, sp, m);
+}
+
+void launchSuperOp(QuregDev *q, const int *t, int k, const Cplx *S) {
+    const int n = q->numQubitsRep;
+    int pos[2 * kPendMax];
+    for (int i = 0; i < k; ++i) {
+        pos[i] = t[i];
+        pos[k + i] = t[i] + n;
+    }
+    SuperSpec sp{};
+    for (int e = 0; e < (1 << (2 * k)); ++e) {
+        uint64_t off = 0;
+        for (int j = 0; j < 2 * k; ++j)
+            if ((e >> j) & 1) off |= 1ull << pos[j];
+        sp.off[e] = off;
+    }
+    std::sort(pos, pos + 2 * k);
+    for (int j = 0; j < 2 * k; ++j) sp.fix[j] = pos[j];
+    const long long groups = q->numAmps >> (2 * k);
+    const bool nt = useNT(sp.fix[0], q->prec);
+    hipStream_t st = hstream();
+    QMX_DISPATCH(q->prec, T, {
+        if (k == 1) launchSuperOpT<T, 1>(q, groups, sp, nt, S, st);
+        else launchSuperOpT<T, 2>(q, groups, sp, nt, S, st);
+    });
+    checkLaunch("density superoperator");
+}
+
+void flushPending() {
+    setPendingWork(nullptr);
+    if (!g_pend.q) return;
+    QuregDev *q = g_pend.q;
+    g_pend.q = nullptr;
+    ++g_passes;
+    launchSuperOp(q, g_pend.t, g_pend.k, g_pend.S);
+    g_pend.k = 0;
+}
+
+// sub-operator s on subT (ks qubits, all in T) written into the 4^K x 4^K
+// space of T: identity on the other qubits' row and column bits
+void embedInto(const int *subT, int ks, const Cplx *s, const int *T, int K, Cplx *out) {
+    const int ds = 1 << (2 * ks), dim = 1 << (2 * K);
+    int p[kPendMax];
+    unsigned subMask = 0;
+    for (int j = 0; j < ks; ++j) {
+        p[j] = (int)(std::find(T, T + K, subT[j]) - T);
+        subMask |= (1u << p[j]) | (1u << (K + p[j]));
+    }
+    auto gather = [&](int X) {
+        int r = 0;
+        for (int j = 0; j < ks; ++j) {
+            r |= ((X >> p[j]) & 1) << j;
+            r |= ((X >> (K + p[j])) & 1) << (ks + j);
+        }
+        return r;
+    };
+    for (int R = 0; R < dim; ++R)
+        for (int C = 0; C < dim; ++C)
+            out[R * dim + C] = ((R ^ C) & ~subMask) ? Cplx{0, 0} : s[gather(R) * ds + gather(C)];
+}
+
+void channelSuperOp(QuregDev *q, int qubit, double f, double d00, double d01, double d10, double d11) {
+    // entry e = row bit + 2 * column bit: e = 0, 3 diagonal, e = 1, 2 off-diagonal
+    Cplx S[16] = {};
+    S[0] = {d00, 0};
+    S[3] = {d01, 0};
+    S[12] = {d10, 0};
+    S[15] = {d11, 0};
+    S[5] = {f, 0};
+    S[10] = {f, 0};
+    densSuperOp(q, &qubit, 1, S);
+}
+
+}  // namespace
+
+void densSuperOp(QuregDev *q, const int *targets, int k, const Cplx *S) {
+    if (k > kPendMax) {
+        std::vector<int> all(2 * k);
+        for (int i = 0; i < k; ++i) {
+            all[i] = targets[i];
+            all[k + i] = targets[i] + q->numQubitsRep;
+        }
+        applyMatrixN(q, 0, all.data(), 2 * k, S);
+        return;
+    }
+    if (g_pend.q && g_pend.q != q) flushPending();
+    int U[2 * kPendMax], K = 0;
+    for (int i = 0; g_pend.q && i < g_pend.k; ++i) U[K++] = g_pend.t[i];
+    for (int j = 0; j < k; ++j)
+        if (std::find(U, U + K, targets[j]) == U + K) U[K++] = targets[j];
+    if (K > kPendMax) {
+        flushPending();
+        K = 0;
+        for (int j = 0; j < k; ++j) U[K++] = targets[j];
+    }
+    const int dim = 1 << (2 * K);
+    Cplx cur[kMaxEntries], add[kMaxEntries];
+    if (g_pend.q) {
+        embedInto(g_pend.t, g_pend.k, g_pend.S, U, K, cur);
+    } else {
+        for (int e = 0; e < dim * dim; ++e) cur[e] = Cplx{0, 0};
+        for (int e = 0; e < dim; ++e) cur[e * dim + e] = Cplx{1, 0};
+    }
+    embedInto(targets, k, S, U, K, add);
+    // later operation on the left
+    for (int r = 0; r < dim; ++r)
+        for (int c = 0; c < dim; ++c) {
+            double re = 0, im = 0;
+            for (int m = 0; m < dim; ++m) {
+                const Cplx a = add[r * dim + m], b = cur[m * dim + c];
+                re += a.re * b.re - a.im * b.im;
+                im += a.re * b.im + a.im * b.re;
+            }
+            g_pend.S[r * dim + c] = Cplx{re, im};
+        }
+    g_pend.q = q;
+    g_pend.k = K;
+    for (int i = 0; i < K; ++i) g_pend.t[i] = U[i];
+    ++g_absorbed;
+    setPendingWork(&flushPending);
+}
+
+void densUnitary(QuregDev *q, const int *targets, int k, const Cplx *u, const Cplx *uc) {
+    // vec(U rho U^dagger) = (conj(U) (x) U) vec(rho): rows take u, columns uc
+    const int d = 1 << k, dim = d * d;
+    std::vector<Cplx> S((size_t)dim * dim);
+    for (int ci = 0; ci < d; ++ci)
+        for (int ri = 0; ri < d; ++ri)
+            for (int cj = 0; cj < d; ++cj)
+                for (int rj = 0; rj < d; ++rj) {
+                    const Cplx a = u[ri * d + rj], b = uc[ci * d + cj];
+                    S[(size_t)(ci * d + ri) * dim + (cj * d + rj)] =
+                        Cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+                }
+    densSuperOp(q, targets, k, S.data());
+}
+
+void densDropPending(const QuregDev *q) {
+    if (g_pend.q == q) {
+        g_pend.q = nullptr;
+        g_pend.k = 0;
+        setPendingWork(nullptr);
+    }
+}
+
+void densSuperOpStats(long long *passes, long long *absorbed) {
+    if (passes) *passes = g_passes;
+    if (absorbed) *absorbed = g_absorbed;
+}
+
+// one-qubit channels as (off-diagonal factor f, diagonal map D) on the
+// (row bit, column bit) quad (QuEST_cpu.c:61-95, 138-235)
+void densDephase(QuregDev *q, int qubit, double retain) { channelSuperOp(q, qubit, retain, 1, 0, 0, 1); }
+
+void densDepolarise(QuregDev *q, int qubit, double depolLevel) {
+    // diag -> (1-d) x + d (x00 + x11)/2
+    const double h = depolLevel / 2;
+    channelSuperOp(q, qubit, 1 - depolLevel, 1 - h, h, h, 1 - h);
+}
+
+void densDamp(QuregDev *q, int qubit, double damping) {
+    // |0><0| += g |1><1|, |1><1| *= 1-g, off-diagonals *= sqrt(1-g)
+    channelSuperOp(q, qubit, std::sqrt(1 - damping), 1, damping, 0, 1 - damping);
 }
 
 }  // namespace qmx

@@ -59,6 +59,9 @@ double *pinnedScratch();            // 64 doubles of pinned host memory
 void checkLaunch(const char *what);
 void timerBegin();
 double timerEndMs();
+// one deferred task, run just before the next use of the stream (launch, copy,
+// synchronisation, timer event); nullptr clears it
+void setPendingWork(void (*fn)());
 int deviceDescription(char *buf, int len);
 
 // ---- state set-up (qmx_state.hip) ------------------------------------------
@@ -127,5 +130,17 @@ void densDephase(QuregDev *q, int qubit, double retain);
 void densTwoQubitDephase(QuregDev *q, int q1, int q2, double retain);
 void densDepolarise(QuregDev *q, int qubit, double depolLevel);
 void densDamp(QuregDev *q, int qubit, double damping);
+
+// Operations on density registers that act on at most two qubits compose on
+// the host into one pending superoperator; it is applied in a single pass over
+// the register when the next operation needs more qubits, another register or
+// the stream. S is row-major over 4^k entries; index bits 0..k-1 are the row
+// bits of targets[0..k-1] and bits k..2k-1 their column bits (rho column-major,
+// element (r, c) at r + c 2^n). More than two targets: applied at once.
+void densSuperOp(QuregDev *q, const int *targets, int k, const Cplx *S);
+// the unitary u (rows) and uc (columns, conj(u) for a unitary) as a superoperator
+void densUnitary(QuregDev *q, const int *targets, int k, const Cplx *u, const Cplx *uc);
+void densDropPending(const QuregDev *q);        // register being destroyed
+void densSuperOpStats(long long *passes, long long *absorbed);
 
 }  // namespace qmx

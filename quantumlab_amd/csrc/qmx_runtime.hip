@@ -15,7 +15,21 @@ hipStream_t g_stream = nullptr;
 double *g_pinned = nullptr;
 hipEvent_t g_ev0 = nullptr, g_ev1 = nullptr;
 hipDeviceProp_t g_prop;
+void (*g_pendingWork)() = nullptr;
+
+// Deferred work (a composed density superoperator) is enqueued before anything
+// else reaches the stream: launches, copies, synchronisation, timer events.
+hipStream_t flushedStream() {
+    if (g_pendingWork) {
+        void (*fn)() = g_pendingWork;
+        g_pendingWork = nullptr;
+        fn();
+    }
+    return g_stream;
+}
 }  // namespace
+
+void setPendingWork(void (*fn)()) { g_pendingWork = fn; }
 
 [[noreturn]] void hipFatal(hipError_t err, const char *what, const char *file, int line) {
     std::fprintf(stderr, "libquest_mi355x: HIP error '%s' (%d) from %s at %s:%d\n",
@@ -46,11 +60,11 @@ void runtimeInit() {
     });
 }
 
-void *stream() { return g_stream; }
+void *stream() { return flushedStream(); }
 
 void deviceSync() {
     runtimeInit();
-    QMX_HIP_CHECK(hipStreamSynchronize(g_stream));
+    QMX_HIP_CHECK(hipStreamSynchronize(flushedStream()));
 }
 
 void *devAlloc(size_t bytes) {
@@ -68,17 +82,17 @@ void *devAlloc(size_t bytes) {
 
 void devFree(void *p) {
     if (!p) return;
-    QMX_HIP_CHECK(hipStreamSynchronize(g_stream));
+    QMX_HIP_CHECK(hipStreamSynchronize(flushedStream()));
     QMX_HIP_CHECK(hipFree(p));
 }
 
 void hostToDevice(void *dst, const void *src, size_t bytes) {
-    QMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    QMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, flushedStream()));
     QMX_HIP_CHECK(hipStreamSynchronize(g_stream));
 }
 
 void deviceToHost(void *dst, const void *src, size_t bytes) {
-    QMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    QMX_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, flushedStream()));
     QMX_HIP_CHECK(hipStreamSynchronize(g_stream));
 }
 
@@ -94,11 +108,11 @@ void checkLaunch(const char *what) {
 
 void timerBegin() {
     runtimeInit();
-    QMX_HIP_CHECK(hipEventRecord(g_ev0, g_stream));
+    QMX_HIP_CHECK(hipEventRecord(g_ev0, flushedStream()));
 }
 
 double timerEndMs() {
-    QMX_HIP_CHECK(hipEventRecord(g_ev1, g_stream));
+    QMX_HIP_CHECK(hipEventRecord(g_ev1, flushedStream()));
     QMX_HIP_CHECK(hipEventSynchronize(g_ev1));
     float ms = 0.f;
     QMX_HIP_CHECK(hipEventElapsedTime(&ms, g_ev0, g_ev1));

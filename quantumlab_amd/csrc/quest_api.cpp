@@ -97,7 +97,29 @@ void rotationPair(double angle, Vector axis, Complex *alpha, Complex *beta) {
 }
 
 // 2x2 on target with controls; density: conj on shifted qubits
+// a one-qubit matrix under one control as a 4x4 matrix on (target, control):
+// index bit 0 is the target, bit 1 the control; active when the control bit
+// equals `on`
+void controlled4(const Cplx u[4], int on, Cplx out[16]) {
+    for (int e = 0; e < 16; ++e) out[e] = Cplx{(e % 5 == 0) ? 1.0 : 0.0, 0};
+    for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) out[(on * 2 + r) * 4 + (on * 2 + c)] = u[r * 2 + c];
+}
+
 void gate2(const Qureg &q, int target, uint64_t ctrl, uint64_t flip, const Cplx u[4], const Cplx uc[4]) {
+    if (q.isDensityMatrix && ctrl == 0) {
+        qmx::densUnitary(D(q), &target, 1, u, uc);
+        return;
+    }
+    if (q.isDensityMatrix && __builtin_popcountll(ctrl) == 1) {
+        int t[2] = {target, __builtin_ctzll(ctrl)};
+        const int on = (flip & ctrl) ? 0 : 1;
+        Cplx u4[16], uc4[16];
+        controlled4(u, on, u4);
+        controlled4(uc, on, uc4);
+        qmx::densUnitary(D(q), t, 2, u4, uc4);
+        return;
+    }
     qmx::applyMatrix2(D(q), target, ctrl, flip, u);
     if (q.isDensityMatrix) {
         int s = q.numQubitsRepresented;
@@ -112,12 +134,36 @@ void gate2Conj(const Qureg &q, int target, uint64_t ctrl, uint64_t flip, const C
 }
 
 void phaseAllOnes(const Qureg &q, uint64_t mask, Cplx term) {
+    if (q.isDensityMatrix && mask && __builtin_popcountll(mask) <= 2) {
+        int t[2], k = 0;
+        for (uint64_t m = mask; m; m &= m - 1) t[k++] = __builtin_ctzll(m);
+        const int d = 1 << k;
+        Cplx u[16] = {}, uc[16] = {};
+        for (int i = 0; i < d; ++i) {
+            u[i * d + i] = (i == d - 1) ? term : Cplx{1, 0};
+            uc[i * d + i] = conjC(u[i * d + i]);
+        }
+        qmx::densUnitary(D(q), t, k, u, uc);
+        return;
+    }
     qmx::applyPhaseOnAllOnes(D(q), mask, term);
     if (q.isDensityMatrix) qmx::applyPhaseOnAllOnes(D(q), mask << q.numQubitsRepresented, conjC(term));
 }
 
 void matrixN(const Qureg &q, uint64_t ctrl, const int *targs, int k, const std::vector<Cplx> &u,
              const std::vector<Cplx> &uc) {
+    if (q.isDensityMatrix && ctrl == 0 && k <= 2) {
+        qmx::densUnitary(D(q), targs, k, u.data(), uc.data());
+        return;
+    }
+    if (q.isDensityMatrix && k == 1 && __builtin_popcountll(ctrl) == 1) {
+        int t[2] = {targs[0], __builtin_ctzll(ctrl)};
+        Cplx u4[16], uc4[16];
+        controlled4(u.data(), 1, u4);
+        controlled4(uc.data(), 1, uc4);
+        qmx::densUnitary(D(q), t, 2, u4, uc4);
+        return;
+    }
     qmx::applyMatrixN(D(q), ctrl, targs, k, u.data());
     if (q.isDensityMatrix) {
         int s = q.numQubitsRepresented;
@@ -228,12 +274,7 @@ std::vector<Cplx> krausSuperOp(int numOps, int d, GetK K) {
 }
 
 void applySuperOp(const Qureg &q, const int *targets, int numTargets, const std::vector<Cplx> &sup) {
-    std::vector<int> all(2 * numTargets);
-    for (int t = 0; t < numTargets; ++t) {
-        all[t] = targets[t];
-        all[t + numTargets] = targets[t] + q.numQubitsRepresented;
-    }
-    qmx::applyMatrixN(D(q), 0, all.data(), 2 * numTargets, sup.data());
+    qmx::densSuperOp(D(q), targets, numTargets, sup.data());
 }
 
 // ---- host mirror allocation ----------------------------------------------------
@@ -456,6 +497,7 @@ void destroyQureg(Qureg qureg, QuESTEnv env) {
     (void)env;
     QuregDev *d = D(qureg);
     if (!d || d->magic != qmx::kMagic) return;
+    qmx::densDropPending(d);
     qmx::devFree(d->amps);
     munmap(qureg.stateVec.real, d->hostBytes);
     munmap(qureg.stateVec.imag, d->hostBytes);
@@ -969,8 +1011,12 @@ void multiStateControlledUnitary(Qureg qureg, int *controlQubits, int *controlSt
 void pauliX(Qureg qureg, int targetQubit) {
     API_TRY
     qv::target(qureg, targetQubit, __func__);
-    qmx::applyNot(D(qureg), 0, 1ull << targetQubit);
-    if (qureg.isDensityMatrix) qmx::applyNot(D(qureg), 0, 1ull << (targetQubit + qureg.numQubitsRepresented));
+    if (qureg.isDensityMatrix) {
+        const Cplx x[4] = {{0, 0}, {1, 0}, {1, 0}, {0, 0}};
+        qmx::densUnitary(D(qureg), &targetQubit, 1, x, x);
+    } else {
+        qmx::applyNot(D(qureg), 0, 1ull << targetQubit);
+    }
     API_CATCH_VOID
 }
 
@@ -1001,10 +1047,14 @@ void hadamard(Qureg qureg, int targetQubit) {
 void controlledNot(Qureg qureg, int controlQubit, int targetQubit) {
     API_TRY
     qv::controlTarget(qureg, controlQubit, targetQubit, __func__);
-    qmx::applyNot(D(qureg), 1ull << controlQubit, 1ull << targetQubit);
     if (qureg.isDensityMatrix) {
-        int s = qureg.numQubitsRepresented;
-        qmx::applyNot(D(qureg), 1ull << (controlQubit + s), 1ull << (targetQubit + s));
+        const Cplx x[4] = {{0, 0}, {1, 0}, {1, 0}, {0, 0}};
+        Cplx u4[16];
+        controlled4(x, 1, u4);
+        int t[2] = {targetQubit, controlQubit};
+        qmx::densUnitary(D(qureg), t, 2, u4, u4);
+    } else {
+        qmx::applyNot(D(qureg), 1ull << controlQubit, 1ull << targetQubit);
     }
     API_CATCH_VOID
 }
@@ -1041,10 +1091,14 @@ void controlledPauliY(Qureg qureg, int controlQubit, int targetQubit) {
 void swapGate(Qureg qureg, int qb1, int qb2) {
     API_TRY
     qv::uniqueTargets(qureg, qb1, qb2, __func__);
-    qmx::applySwap(D(qureg), qb1, qb2);
     if (qureg.isDensityMatrix) {
-        int s = qureg.numQubitsRepresented;
-        qmx::applySwap(D(qureg), qb1 + s, qb2 + s);
+        Cplx sw[16] = {};
+        sw[0] = sw[15] = Cplx{1, 0};
+        sw[1 * 4 + 2] = sw[2 * 4 + 1] = Cplx{1, 0};
+        int t[2] = {qb1, qb2};
+        qmx::densUnitary(D(qureg), t, 2, sw, sw);
+    } else {
+        qmx::applySwap(D(qureg), qb1, qb2);
     }
     API_CATCH_VOID
 }
@@ -1637,6 +1691,7 @@ int qmxLastError(char *buf, int len) { return qv::lastError(buf, len); }
 void qmxClearError(void) { qv::clear(); }
 int qmxDeviceInfo(char *buf, int len) { return qmx::deviceDescription(buf, len); }
 void qmxSync(void) { qmx::deviceSync(); }
+void qmxDensityFusionStats(long long *sweeps, long long *operations) { qmx::densSuperOpStats(sweeps, operations); }
 void qmxTimerBegin(void) { qmx::timerBegin(); }
 double qmxTimerEndMs(void) { return qmx::timerEndMs(); }
 

@@ -235,6 +235,7 @@ _EXT_SIGS = {
     "qmxClearError": (None, []),
     "qmxDeviceInfo": (C.c_int, [C.c_char_p, C.c_int]),
     "qmxSync": (None, []),
+    "qmxDensityFusionStats": (None, [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "qmxTimerBegin": (None, []),
     "qmxTimerEndMs": (C.c_double, []),
     "qmxTimerSlotBegin": (None, [C.c_int]),

@@ -274,6 +274,55 @@ def test_cpuref_channels_match_kraus_sums(cpuref):
     cpuref.destroy(rhoq)
 
 
+def test_cpuref_two_qubit_channels_match_kraus_sums(cpuref):
+    from quantumlab_amd.quest import ComplexMatrix4, matrix4
+    n = 3
+    rng = np.random.default_rng(73)
+    pure = random_state(n, 74)
+    rho, rhoq = cpuref.qureg(n, density=True), cpuref.qureg(n)
+    cpuref.set_state(rhoq, pure)
+    cpuref.initPureState(rho, rhoq)
+    r = np.outer(pure, pure.conj())
+    p = 0.37
+    cpuref.call("mixTwoQubitDepolarising", rho, 2, 0, p)
+    paulis = [D.I2, D.X, D.Y, D.Z]
+    ops = [(np.sqrt(1 - p) if a == b == 0 else np.sqrt(p / 15)) * D.full_op(n, [2], paulis[a]) @ D.full_op(n, [0], paulis[b])
+           for a in range(4) for b in range(4)]
+    r = D.kraus(r, ops)
+    assert np.max(np.abs(D.to_rho(cpuref.state(rho)) - r)) < TOL
+    # a random 2-operator CPTP map on (1, 2) from a 8x4 isometry; targets[0] is the low index bit
+    V = W.haar_unitary(rng, 8)[:, :4]
+    K = [V[:4, :], V[4:, :]]
+    cpuref.call("mixTwoQubitKrausMap", rho, 1, 2, (ComplexMatrix4 * 2)(matrix4(K[0]), matrix4(K[1])), 2)
+    r = D.kraus(r, [D.full_op(n, [1, 2], k) for k in K])
+    assert np.max(np.abs(D.to_rho(cpuref.state(rho)) - r)) < TOL
+    for row, col in ((0, 0), (3, 5), (7, 2)):
+        z = cpuref.getDensityAmp(rho, row, col)
+        assert abs(complex(z.real, z.imag) - r[row, col]) < TOL
+    cpuref.destroy(rho)
+    cpuref.destroy(rhoq)
+
+
+def test_cpuref_two_qubit_channels_equal_reference(cpuref, reflib):
+    from quantumlab_amd.quest import ComplexMatrix4, matrix4
+    n = 4
+    rng = np.random.default_rng(75)
+    V = W.haar_unitary(rng, 12)[:, :4]
+    K = (ComplexMatrix4 * 3)(*[matrix4(V[4 * i:4 * i + 4, :]) for i in range(3)])
+    out = []
+    for lib in (cpuref, reflib):
+        rho, pure = lib.qureg(n, density=True), lib.qureg(n)
+        channel_circuit(lib, rho, pure, n, 2, seed=76)
+        lib.call("mixTwoQubitDepolarising", rho, 3, 1, 0.4)
+        lib.call("mixTwoQubitKrausMap", rho, 0, 2, K, 3)
+        z = lib.getDensityAmp(rho, 5, 11)
+        out.append((lib.state(rho), complex(z.real, z.imag)))
+        lib.destroy(rho)
+        lib.destroy(pure)
+    assert np.max(np.abs(out[0][0] - out[1][0])) < 1e-13
+    assert abs(out[0][1] - out[1][1]) < 1e-13
+
+
 # ---- 4. known answers and closed forms ---------------------------------------------
 
 def test_cpuref_period_one_hamiltonian_known_answer(cpuref):
