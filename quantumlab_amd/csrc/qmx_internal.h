@@ -63,6 +63,7 @@ double timerEndMs();
 // synchronisation, timer event); nullptr clears it
 void setPendingWork(void (*fn)());
 int deviceDescription(char *buf, int len);
+int numCUs();                       // compute units of the device
 
 // ---- state set-up (qmx_state.hip) ------------------------------------------
 void initBlank(QuregDev *q);

@@ -119,6 +119,11 @@ double timerEndMs() {
     return (double)ms;
 }
 
+int numCUs() {
+    runtimeInit();
+    return g_prop.multiProcessorCount;
+}
+
 int deviceDescription(char *buf, int len) {
     runtimeInit();
     size_t freeB = 0, totalB = 0;
