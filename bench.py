@@ -305,7 +305,7 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                          "sweeps": int(sum(x[6] for x in rows)), "bytes_per_sweep": 2.0 * (1 << n) * 8,
                          "note": "read + write of the FP32 register per fused block, over the learned solves' wall "
-                                 "time; the in-tile rotation arithmetic (VALU) is what holds it below HBM"},
+                                 "time; tile load latency and the per-pass LDS round trips hold it below HBM"},
             "choices": [[x[1], x[2]] for x in rows],
             "step_doubling_reps": [[x[5][2][1], x[5][4][1]] for x in rows]}, rows
 
