@@ -55,6 +55,23 @@ double qmxTimerSlotMs(int slot);
  * register or the stream. Cumulative counts since load: sweeps made and
  * operations absorbed into them. */
 void qmxDensityFusionStats(long long *sweeps, long long *operations);
+/* QuantumLab's amplitude-encoding circuit for the real vector vec (2^numQubits
+ * entries, encoded on qubits 0..numQubits-1, bit l <-> qubit l) applied to
+ * |0...0>: the register ends in vec/|vec| (|0...0> for a zero vector), as
+ * the reference's Ry tree + Gray-code ladder prepares it (angles and product
+ * in FP64), in one launch instead of 2^numQubits gates. State vectors, at most
+ * 12 encoded qubits; the register's other qubits are left |0>. */
+void qmxAmplitudeEncodeReal(Qureg qureg, const qreal *vec, int numQubits);
+/* Repeated measurement of qubits[0..numQubits) in that order, as a loop that
+ * restores the register and calls measure() on each qubit per shot would do
+ * (QuantumLab's QLSimulatorMeasure.cpp:113-140), without touching the state:
+ * one joint-probability reduction on the device, then every shot's chain of
+ * outcomes drawn on the host from QuEST's generator (same draws, same order).
+ * outcomes[s] has bit j set when qubits[j] measured 1. untilCount > 0 stops
+ * after the first shot whose outcome has then occurred untilCount times
+ * (the loop's m_iMeasureUntil mode). Returns the number of shots drawn.
+ * At most 26 qubits. */
+int qmxSampleShots(Qureg qureg, const int *qubits, int numQubits, int shots, int untilCount, int *outcomes);
 /* max |a_i - b_i| over amplitudes (any precisions, same size) */
 qreal qmxMaxAbsDiff(Qureg a, Qureg b);
 /* FP64 norm of the register; if rescale != 0 the register is renormalised */

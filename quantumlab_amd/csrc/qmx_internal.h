@@ -64,6 +64,9 @@ double timerEndMs();
 void setPendingWork(void (*fn)());
 int deviceDescription(char *buf, int len);
 int numCUs();                       // compute units of the device
+// raise a kernel's dynamic-LDS limit to `bytes` (beyond the 64 KiB default);
+// false, with a message, when the device has less
+bool allowDynamicLds(const void *kernel, size_t bytes, const char *caller);
 
 // ---- state set-up (qmx_state.hip) ------------------------------------------
 void initBlank(QuregDev *q);
@@ -109,6 +112,10 @@ struct PauliExp {
 };
 void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps);
 int pauliSequenceBlocks(int numQubits, int prec, const PauliExp *seq, int count);
+
+// amplitude-encoding circuit of `vec` (2^qbits reals, qbits <= 12) applied to
+// |0...0> of the register (qmx_swaptest.hip)
+void amplitudeEncode(QuregDev *q, const double *vec, int qbits);
 
 // ---- reductions (qmx_reduce.hip); all accumulate in FP64 -------------------
 double probOfZero(const QuregDev *q, int qubit);                   // state vector

@@ -119,6 +119,17 @@ double timerEndMs() {
     return (double)ms;
 }
 
+bool allowDynamicLds(const void *kernel, size_t bytes, const char *caller) {
+    runtimeInit();
+    if (bytes + 1024 > g_prop.sharedMemPerBlock) {
+        std::fprintf(stderr, "libquest_mi355x: %s: %zu B of LDS exceeds the device's %zu B\n", caller, bytes,
+                     (size_t)g_prop.sharedMemPerBlock);
+        return false;
+    }
+    QMX_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return true;
+}
+
 int numCUs() {
     runtimeInit();
     return g_prop.multiProcessorCount;

@@ -494,6 +494,46 @@ struct qmxSwapPlan {
     SwapPlanImpl impl;
 };
 
+namespace qmx {
+namespace {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kAEWrite(Amp<T> *__restrict__ a, const double *__restrict__ v, int D) {
+    const int x = blockIdx.x * kBlock + threadIdx.x;
+    if (x < D) a[x] = Amp<T>{(T)v[x], (T)0};
+}
+}  // namespace
+
+// the reference's amplitude-encoding circuit (AmplitudeEncodeCuda.cu:112-150 angles,
+// Ry + uniformly controlled Ry ladder, FRGate.cu:181-252) applied to |0...0>:
+// qubits 0..q-1 take the encoded vector, the others stay |0>
+void amplitudeEncode(QuregDev *q, const double *vec, int qbits) {
+    const int D = 1 << qbits;
+    const size_t smem = (size_t)3 * D * sizeof(double);
+    if (smem > 65536 && !allowDynamicLds(reinterpret_cast<const void *>(kAEState<double>), smem,
+                                         "qmxAmplitudeEncodeReal"))
+        return;
+    struct Hdr {
+        long long off;
+        int q;
+    };
+    char *buf = static_cast<char *>(devAlloc((size_t)2 * D * sizeof(double) + 64));
+    double *dIn = reinterpret_cast<double *>(buf), *dOut = dIn + D;
+    long long *dOff = reinterpret_cast<long long *>(dOut + D);
+    int *dQ = reinterpret_cast<int *>(dOff + 1);
+    const Hdr h{0, qbits};
+    hostToDevice(dIn, vec, (size_t)D * sizeof(double));
+    hostToDevice(dOff, &h.off, sizeof(long long));
+    hostToDevice(dQ, &h.q, sizeof(int));
+    hipLaunchKernelGGL(kAEState<double>, dim3(1), dim3(kBlock), smem, hstream(), dIn, dOff, dQ, dOut, 1);
+    initBlank(q);
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kAEWrite<T>, dim3(gridFor(D, 1)), dim3(kBlock), 0, hstream(), (Amp<T> *)q->amps, dOut, D);
+    });
+    checkLaunch("qmxAmplitudeEncodeReal");
+    devFree(buf);
+}
+}  // namespace qmx
+
 extern "C" {
 
 qmxSwapPlan *qmxSwapPlanCreate(int numPoints, const int *pointQubits, const double *points, int numCentroids,
@@ -576,6 +616,16 @@ qmxSwapPlan *qmxSwapPlanCreate(int numPoints, const int *pointQubits, const doub
     hostToDevice(s->pQ, s->hQ.data(), numPoints * sizeof(int));
     hostToDevice(s->cQ, cq.data(), cq.size() * sizeof(int));
     s->aeSmem = (size_t)3 * (1u << s->qHi) * sizeof(double);
+    // 11- and 12-qubit registers need more than the 64 KiB default of dynamic LDS
+    bool ldsOk = true;
+    QMX_DISPATCH(precision, T, {
+        if (s->aeSmem > 65536)
+            ldsOk = allowDynamicLds(reinterpret_cast<const void *>(kAEState<T>), s->aeSmem, "qmxSwapPlanCreate");
+    });
+    if (!ldsOk) {
+        qmxSwapPlanDestroy(plan);
+        return nullptr;
+    }
     // default tile rows: FP32 (one wave per tile) 64 rows, or the register
     // when smaller; FP64 (one workgroup per tile) ~16K amplitudes a tile
     for (int q = 0; q <= kMaxQ; ++q)

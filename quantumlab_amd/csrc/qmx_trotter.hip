@@ -290,20 +290,6 @@ bool pack(int numJobs, const PauliHamil *hamils, const double *times, const doub
     return true;
 }
 
-// registers up to 12 qubits need 64 KiB of dynamic LDS plus the static
-// arrays; raise the kernel's dynamic-LDS limit to what this device allows
-bool allowLds(const void *kernel, size_t bytes, const char *caller) {
-    hipDeviceProp_t prop;
-    QMX_HIP_CHECK(hipGetDeviceProperties(&prop, 0));
-    if (bytes + 1024 > prop.sharedMemPerBlock) {
-        std::fprintf(stderr, "libquest_mi355x: %s: %zu B of LDS exceeds the device's %zu B\n", caller, bytes,
-                     (size_t)prop.sharedMemPerBlock);
-        return false;
-    }
-    QMX_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    return true;
-}
-
 template <typename V>
 V *upload(const std::vector<V> &v) {
     V *d = static_cast<V *>(devAlloc(std::max<size_t>(1, v.size()) * sizeof(V)));
@@ -336,7 +322,7 @@ int qmxTrotterBatchEvolve(int numJobs, const PauliHamil *hamils, const qreal *ti
     int maxN = 1;
     for (auto &b : pk.jobs) maxN = std::max(maxN, b.n);
     const size_t lds = (size_t)(1u << maxN) * sizeof(double2);
-    if (!allowLds(reinterpret_cast<const void *>(kTrotterEvolve), lds, "qmxTrotterBatchEvolve")) return 0;
+    if (!allowDynamicLds(reinterpret_cast<const void *>(kTrotterEvolve), lds, "qmxTrotterBatchEvolve")) return 0;
     BatchJob *dJobs = upload(pk.jobs);
     PTerm *dTerms = upload(pk.terms);
     Stage *dStages = upload(pk.stages);
@@ -370,7 +356,7 @@ int qmxTrotterBatchInfidelity(int numJobs, const PauliHamil *hamils, const qreal
         pk.jobs[j].reps = reps[j];
         allN = std::max(allN, pk.jobs[j].n);
     }
-    if (!allowLds(reinterpret_cast<const void *>(kTrotterInfidelity), (size_t)(1u << allN) * sizeof(double2),
+    if (!allowDynamicLds(reinterpret_cast<const void *>(kTrotterInfidelity), (size_t)(1u << allN) * sizeof(double2),
                   "qmxTrotterBatchInfidelity"))
         return 0;
     PTerm *dTerms = upload(pk.terms);
@@ -417,7 +403,7 @@ int qmxTrotterBatchLadder(int numJobs, const PauliHamil *hamils, const qreal *ti
     const int stride = maxLog2 + 1;
     int allN = 1;
     for (auto &b : pk.jobs) allN = std::max(allN, b.n);
-    if (!allowLds(reinterpret_cast<const void *>(kTrotterLadder), (size_t)(1u << allN) * sizeof(double2),
+    if (!allowDynamicLds(reinterpret_cast<const void *>(kTrotterLadder), (size_t)(1u << allN) * sizeof(double2),
                   "qmxTrotterBatchLadder"))
         return 0;
     for (long long i = 0; i < (long long)numJobs * stride; ++i) infidelity[i] = NAN;

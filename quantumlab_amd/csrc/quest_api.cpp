@@ -1692,6 +1692,65 @@ void qmxClearError(void) { qv::clear(); }
 int qmxDeviceInfo(char *buf, int len) { return qmx::deviceDescription(buf, len); }
 void qmxSync(void) { qmx::deviceSync(); }
 void qmxDensityFusionStats(long long *sweeps, long long *operations) { qmx::densSuperOpStats(sweeps, operations); }
+
+// QuantumLab's amplitude encoding (AmplitudeEncodeCuda.cu:498-533: angle tree,
+// then an Ry and a Gray-code ladder of CNOT + Ry per level, one gate launch
+// each) as one state preparation: the product of the tree's cos/sin factors
+// along each basis state's path, written into qubits 0..numQubits-1.
+void qmxAmplitudeEncodeReal(Qureg qureg, const qreal *vec, int numQubits) {
+    API_TRY
+    qv::stateVecQureg(qureg, __func__);
+    if (numQubits < 1 || numQubits > 12 || numQubits > qureg.numQubitsRepresented)
+        qv::raise("Invalid number of qubits to encode. Must be >0, <=12 and <=numQubits.", __func__);
+    if (!vec) qv::raise("The vector is null.", __func__);
+    qmx::amplitudeEncode(D(qureg), vec, numQubits);
+    API_CATCH_VOID
+}
+
+// Repeated measurement without re-uploading the state. QuantumLab's shot loop
+// (QLSimulatorMeasure.cpp:113-140) restores the register from the host before
+// every shot and calls measure() on each listed qubit in order, so shot s is a
+// chain of conditional outcomes, each drawn by generateMeasurementOutcome from
+// QuEST's generator. Here the joint distribution of the listed qubits is
+// reduced once on the device (calcProbOfAllOutcomes), and each shot replays
+// that chain on the host: P(qubit j = 0 | earlier outcomes) is the partial sum
+// of the entries that extend the prefix, scaled by 1 / (product of the earlier
+// outcome probabilities) exactly as collapse-and-renormalise scales the state.
+// The same draws are consumed, so outcomes and the generator's later state
+// match the loop's (a draw could only differ if it fell within rounding of P0).
+int qmxSampleShots(Qureg qureg, const int *qubits, int numQubits, int shots, int untilCount, int *outcomes) {
+    API_TRY
+    qv::multiTargets(qureg, qubits, numQubits, __func__);
+    if (numQubits > 26) qv::raise("At most 26 qubits may be sampled jointly.", __func__);
+    if (shots < 0) qv::raise("The number of shots must be >= 0.", __func__);
+    if (shots > 0 && !outcomes) qv::raise("The outcome array is null.", __func__);
+    const int k = numQubits;
+    // level d holds, for every setting of qubits[0..d-1], the probability mass
+    // of the entries that extend it (index bit j = outcome of qubits[j])
+    std::vector<std::vector<double>> level(k + 1);
+    level[k].resize((size_t)1 << k);
+    qmx::probOfAllOutcomes(D(qureg), qubits, k, level[k].data());
+    for (int d = k - 1; d >= 0; --d) {
+        level[d].resize((size_t)1 << d);
+        for (size_t p = 0; p < level[d].size(); ++p) level[d][p] = level[d + 1][p] + level[d + 1][p | ((size_t)1 << d)];
+    }
+    std::vector<long long> counts(untilCount > 0 ? ((size_t)1 << k) : 0, 0);
+    int drawn = 0;
+    while (drawn < shots) {
+        size_t prefix = 0;
+        double scale = 1.0;
+        for (int j = 0; j < k; ++j) {
+            double outcomeProb;
+            const int o = generateMeasurementOutcome(level[j + 1][prefix] * scale, &outcomeProb);
+            prefix |= (size_t)o << j;
+            scale /= outcomeProb;
+        }
+        outcomes[drawn++] = (int)prefix;
+        if (untilCount > 0 && ++counts[prefix] >= untilCount) break;
+    }
+    return drawn;
+    API_CATCH(0)
+}
 void qmxTimerBegin(void) { qmx::timerBegin(); }
 double qmxTimerEndMs(void) { return qmx::timerEndMs(); }
 

@@ -236,6 +236,8 @@ _EXT_SIGS = {
     "qmxDeviceInfo": (C.c_int, [C.c_char_p, C.c_int]),
     "qmxSync": (None, []),
     "qmxDensityFusionStats": (None, [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "qmxSampleShots": (C.c_int, [Qureg, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "qmxAmplitudeEncodeReal": (None, [Qureg, C.POINTER(qreal), C.c_int]),
     "qmxTimerBegin": (None, []),
     "qmxTimerEndMs": (C.c_double, []),
     "qmxTimerSlotBegin": (None, [C.c_int]),
