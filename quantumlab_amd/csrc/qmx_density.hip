@@ -152,11 +152,17 @@ constexpr int kMaxEntries = 256;           // (4^2)^2
 constexpr int kRing = 256;                 // matrix slots in flight
 constexpr size_t kSlotBytes = kMaxEntries * sizeof(Amp<double>);
 
-struct Pending {
-    QuregDev *q = nullptr;
+// Pending work of one register: superoperators on disjoint qubit sets of at
+// most two qubits each, in creation order. Blocks on disjoint qubits commute,
+// so an operation may join the one block it touches while others stay pending.
+struct Block {
     int k = 0;
     int t[kPendMax] = {0, 0};
     Cplx S[kMaxEntries];
+};
+struct Pending {
+    QuregDev *q = nullptr;
+    std::vector<Block> blocks;
 };
 Pending g_pend;
 long long g_passes = 0, g_absorbed = 0;
@@ -232,14 +238,19 @@ void launchSuperOp(QuregDev *q, const int *t, int k, const Cplx *S) {
     checkLaunch("density superoperator");
 }
 
+void sweepBlock(QuregDev *q, const Block &b) {
+    ++g_passes;
+    launchSuperOp(q, b.t, b.k, b.S);
+}
+
 void flushPending() {
     setPendingWork(nullptr);
     if (!g_pend.q) return;
     QuregDev *q = g_pend.q;
+    std::vector<Block> blocks;
+    blocks.swap(g_pend.blocks);
     g_pend.q = nullptr;
-    ++g_passes;
-    launchSuperOp(q, g_pend.t, g_pend.k, g_pend.S);
-    g_pend.k = 0;
+    for (const Block &b : blocks) sweepBlock(q, b);
 }
 
 // sub-operator s on subT (ks qubits, all in T) written into the 4^K x 4^K
@@ -281,6 +292,7 @@ void channelSuperOp(QuregDev *q, int qubit, double f, double d00, double d01, do
 
 void densSuperOp(QuregDev *q, const int *targets, int k, const Cplx *S) {
     if (k > kPendMax) {
+        flushPending();
         std::vector<int> all(2 * k);
         for (int i = 0; i < k; ++i) {
             all[i] = targets[i];
@@ -289,39 +301,75 @@ void densSuperOp(QuregDev *q, const int *targets, int k, const Cplx *S) {
         applyMatrixN(q, 0, all.data(), 2 * k, S);
         return;
     }
+    // sweeps below launch directly: the stream must not run the pending work
+    // (these very blocks) on their behalf; the hook is set again on return
+    setPendingWork(nullptr);
     if (g_pend.q && g_pend.q != q) flushPending();
-    int U[2 * kPendMax], K = 0;
-    for (int i = 0; g_pend.q && i < g_pend.k; ++i) U[K++] = g_pend.t[i];
-    for (int j = 0; j < k; ++j)
-        if (std::find(U, U + K, targets[j]) == U + K) U[K++] = targets[j];
-    if (K > kPendMax) {
-        flushPending();
-        K = 0;
-        for (int j = 0; j < k; ++j) U[K++] = targets[j];
-    }
-    const int dim = 1 << (2 * K);
-    Cplx cur[kMaxEntries], add[kMaxEntries];
-    if (g_pend.q) {
-        embedInto(g_pend.t, g_pend.k, g_pend.S, U, K, cur);
-    } else {
-        for (int e = 0; e < dim * dim; ++e) cur[e] = Cplx{0, 0};
-        for (int e = 0; e < dim; ++e) cur[e * dim + e] = Cplx{1, 0};
-    }
-    embedInto(targets, k, S, U, K, add);
-    // later operation on the left
-    for (int r = 0; r < dim; ++r)
-        for (int c = 0; c < dim; ++c) {
-            double re = 0, im = 0;
-            for (int m = 0; m < dim; ++m) {
-                const Cplx a = add[r * dim + m], b = cur[m * dim + c];
-                re += a.re * b.re - a.im * b.im;
-                im += a.re * b.im + a.im * b.re;
-            }
-            g_pend.S[r * dim + c] = Cplx{re, im};
-        }
     g_pend.q = q;
-    g_pend.k = K;
-    for (int i = 0; i < K; ++i) g_pend.t[i] = U[i];
+    std::vector<Block> &bl = g_pend.blocks;
+    auto touches = [&](const Block &b) {
+        for (int i = 0; i < b.k; ++i)
+            for (int j = 0; j < k; ++j)
+                if (b.t[i] == targets[j]) return true;
+        return false;
+    };
+    int hit = -1, hits = 0;
+    for (int i = 0; i < (int)bl.size(); ++i)
+        if (touches(bl[i])) {
+            hit = i;
+            ++hits;
+        }
+    int into = -1;
+    if (hits == 1) {
+        int u = bl[hit].k;
+        for (int j = 0; j < k; ++j)
+            if (std::find(bl[hit].t, bl[hit].t + bl[hit].k, targets[j]) == bl[hit].t + bl[hit].k) ++u;
+        if (u <= kPendMax) into = hit;
+    } else if (hits == 0 && k == 1) {
+        // pair a lone qubit with the latest one-qubit block: one sweep for both
+        for (int i = (int)bl.size() - 1; i >= 0 && into < 0; --i)
+            if (bl[i].k == 1) into = i;
+    }
+    if (into < 0 && hits > 0) {
+        // the blocks this operation would bridge are swept first, in order
+        std::vector<Block> keep;
+        for (Block &b : bl) {
+            if (touches(b)) sweepBlock(q, b);
+            else keep.push_back(b);
+        }
+        bl.swap(keep);
+    }
+    if (into < 0) {
+        bl.emplace_back();
+        Block &b = bl.back();
+        b.k = k;
+        for (int j = 0; j < k; ++j) b.t[j] = targets[j];
+        const int dim = 1 << (2 * k);
+        for (int e = 0; e < dim * dim; ++e) b.S[e] = S[e];
+    } else {
+        Block &b = bl[into];
+        int U[2 * kPendMax], K = 0;
+        for (int i = 0; i < b.k; ++i) U[K++] = b.t[i];
+        for (int j = 0; j < k; ++j)
+            if (std::find(U, U + K, targets[j]) == U + K) U[K++] = targets[j];
+        const int dim = 1 << (2 * K);
+        std::vector<Cplx> cur((size_t)dim * dim), add((size_t)dim * dim);
+        embedInto(b.t, b.k, b.S, U, K, cur.data());
+        embedInto(targets, k, S, U, K, add.data());
+        // later operation on the left
+        for (int r = 0; r < dim; ++r)
+            for (int c = 0; c < dim; ++c) {
+                double re = 0, im = 0;
+                for (int m = 0; m < dim; ++m) {
+                    const Cplx x = add[(size_t)r * dim + m], y = cur[(size_t)m * dim + c];
+                    re += x.re * y.re - x.im * y.im;
+                    im += x.re * y.im + x.im * y.re;
+                }
+                b.S[r * dim + c] = Cplx{re, im};
+            }
+        b.k = K;
+        for (int i = 0; i < K; ++i) b.t[i] = U[i];
+    }
     ++g_absorbed;
     setPendingWork(&flushPending);
 }
@@ -344,7 +392,7 @@ void densUnitary(QuregDev *q, const int *targets, int k, const Cplx *u, const Cp
 void densDropPending(const QuregDev *q) {
     if (g_pend.q == q) {
         g_pend.q = nullptr;
-        g_pend.k = 0;
+        g_pend.blocks.clear();
         setPendingWork(nullptr);
     }
 }

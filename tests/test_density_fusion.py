@@ -1,6 +1,6 @@
 """Density registers compose every operation on at most two qubits (gates,
-controlled gates, channels, Kraus maps) into one pending superoperator and
-sweep the register once per composed run. These tests drive interleavings that
+controlled gates, channels, Kraus maps) into pending superoperators on
+disjoint qubit pairs and sweep the register once per composed block. These tests drive interleavings that
 stress that deferral: runs that grow from one to two qubits, operations on a
 third qubit that force a sweep, operations the composition does not take
 (three-qubit gates, two-qubit dephasing), reads in the middle of a run, two
@@ -162,22 +162,31 @@ def composed_targets(op):
 
 
 def expected_sweeps(ops):
-    """The composition rule: a run grows while its qubits number at most two;
-    an operation that would make three, an operation that is not composed, or
-    a read ends the run with one sweep. The final state read ends the last."""
-    pending, sweeps = [], 0
+    """The composition rule. Pending blocks act on disjoint sets of at most two
+    qubits. An operation joins the one block it touches if the union stays
+    within two qubits; a one-qubit operation touching none joins the latest
+    one-qubit block; otherwise the blocks it touches are swept and it opens a
+    new block. An operation that is not composed, or a read, sweeps every
+    block, and so does the final state read."""
+    blocks, sweeps = [], 0
     for op in ops:
         t = composed_targets(op)
         if t is None:
-            sweeps += bool(pending)
-            pending = []
+            sweeps += len(blocks)
+            blocks = []
             continue
-        grown = pending + [x for x in t if x not in pending]
-        if len(grown) > 2:
-            sweeps += 1
-            grown = list(t)
-        pending = grown
-    return sweeps + bool(pending)
+        hit = [i for i, b in enumerate(blocks) if set(b) & set(t)]
+        into = None
+        if len(hit) == 1 and len(set(blocks[hit[0]]) | set(t)) <= 2:
+            into = hit[0]
+        elif not hit and len(t) == 1:
+            into = next((i for i in range(len(blocks) - 1, -1, -1) if len(blocks[i]) == 1), None)
+        if into is None:
+            sweeps += len(hit)
+            blocks = [b for i, b in enumerate(blocks) if i not in hit] + [list(t)]
+        else:
+            blocks[into] = blocks[into] + [x for x in t if x not in blocks[into]]
+    return sweeps + len(blocks)
 
 
 def fusion_stats(gpu):
