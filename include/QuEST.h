@@ -19,8 +19,7 @@
  *  - work is queued on one HIP stream per process; every calc* returns a host
  *    value and therefore synchronises.
  *
- * Functions of the reference API that are outside this library's scope
- * (DiagonalOp, phase functions, QFT, QASM text output) are not declared here.
+ * QASM recording calls are accepted and record nothing (no QASM text output).
  */
 #ifndef QUEST_H
 #define QUEST_H
@@ -108,6 +107,22 @@ typedef struct PauliHamil {
     int numSumTerms;
     int numQubits;
 } PauliHamil;
+
+/* QuEST.h:329-345. Field use in this library:
+ *   real, imag         host copy of the 2^numQubits diagonal (calloc'ed)
+ *   deviceOperator     .real / .imag = device copies (FP64), kept in step by
+ *                      initDiagonalOp, setDiagonalOpElems,
+ *                      initDiagonalOpFromPauliHamil and syncDiagonalOp
+ */
+typedef struct DiagonalOp {
+    int numQubits;
+    long long int numElemsPerChunk;
+    int numChunks;
+    int chunkId;
+    qreal *real;
+    qreal *imag;
+    ComplexArray deviceOperator;
+} DiagonalOp;
 
 /* QuEST.h:354-385. Field use in this library:
  *   stateVec           host mirror (SoA, qreal), lazily committed pages
@@ -278,6 +293,60 @@ void QUESTAPI mixMultiQubitKrausMap(Qureg qureg, int *targets, int numTargets, C
 void QUESTAPI mixNonTPKrausMap(Qureg qureg, int target, ComplexMatrix2 *ops, int numOps);    /* :5119 */
 void QUESTAPI mixNonTPTwoQubitKrausMap(Qureg qureg, int target1, int target2, ComplexMatrix4 *ops, int numOps); /* :5158 */
 void QUESTAPI mixNonTPMultiQubitKrausMap(Qureg qureg, int *targets, int numTargets, ComplexMatrixN *ops, int numOps); /* :5209 */
+
+/* ---- diagonal operators (QuEST.h:1014-1292) -------------------------------- */
+DiagonalOp QUESTAPI createDiagonalOp(int numQubits, QuESTEnv env);                                /* :1014 */
+void QUESTAPI destroyDiagonalOp(DiagonalOp op, QuESTEnv env);                                      /* :1028 */
+void QUESTAPI syncDiagonalOp(DiagonalOp op);                                                       /* :1048 */
+void QUESTAPI initDiagonalOp(DiagonalOp op, qreal *real, qreal *imag);                              /* :1076 */
+void QUESTAPI initDiagonalOpFromPauliHamil(DiagonalOp op, PauliHamil hamil);                       /* :1130 */
+DiagonalOp QUESTAPI createDiagonalOpFromPauliHamilFile(char *fn, QuESTEnv env);                    /* :1174 */
+void QUESTAPI setDiagonalOpElems(DiagonalOp op, long long int startInd, qreal *real, qreal *imag,
+                                 long long int numElems);                                          /* :1222 */
+void QUESTAPI applyDiagonalOp(Qureg qureg, DiagonalOp op);                                         /* :1260 */
+Complex QUESTAPI calcExpecDiagonalOp(Qureg qureg, DiagonalOp op);                                  /* :1292 */
+
+/* ---- phase functions and QFT (QuEST.h:5904-6874) --------------------------- */
+void QUESTAPI applyPhaseFunc(Qureg qureg, int *qubits, int numQubits, enum bitEncoding encoding, qreal *coeffs,
+                             qreal *exponents, int numTerms);                                      /* :5904 */
+void QUESTAPI applyPhaseFuncOverrides(Qureg qureg, int *qubits, int numQubits, enum bitEncoding encoding,
+                                      qreal *coeffs, qreal *exponents, int numTerms, long long int *overrideInds,
+                                      qreal *overridePhases, int numOverrides);                    /* :6015 */
+void QUESTAPI applyMultiVarPhaseFunc(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs,
+                                     enum bitEncoding encoding, qreal *coeffs, qreal *exponents,
+                                     int *numTermsPerReg);                                         /* :6176 */
+void QUESTAPI applyMultiVarPhaseFuncOverrides(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs,
+                                              enum bitEncoding encoding, qreal *coeffs, qreal *exponents,
+                                              int *numTermsPerReg, long long int *overrideInds,
+                                              qreal *overridePhases, int numOverrides);            /* :6258 */
+void QUESTAPI applyNamedPhaseFunc(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs,
+                                  enum bitEncoding encoding, enum phaseFunc functionNameCode);     /* :6398 */
+void QUESTAPI applyNamedPhaseFuncOverrides(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs,
+                                           enum bitEncoding encoding, enum phaseFunc functionNameCode,
+                                           long long int *overrideInds, qreal *overridePhases,
+                                           int numOverrides);                                      /* :6471 */
+void QUESTAPI applyParamNamedPhaseFunc(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs,
+                                       enum bitEncoding encoding, enum phaseFunc functionNameCode, qreal *params,
+                                       int numParams);                                             /* :6589 */
+void QUESTAPI applyParamNamedPhaseFuncOverrides(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs,
+                                                enum bitEncoding encoding, enum phaseFunc functionNameCode,
+                                                qreal *params, int numParams, long long int *overrideInds,
+                                                qreal *overridePhases, int numOverrides);          /* :6664 */
+void QUESTAPI applyFullQFT(Qureg qureg);                                                           /* :6758 */
+void QUESTAPI applyQFT(Qureg qureg, int *qubits, int numQubits);                                   /* :6874 */
+
+/* ---- C-only helpers for variable-length 2D arrays (QuEST.h:800, 5729-5795) -- */
+#ifndef __cplusplus
+#include <stddef.h> /* NULL in getStaticComplexMatrixN */
+void QUESTAPI initComplexMatrixN(ComplexMatrixN m, qreal real[][1 << m.numQubits], qreal imag[][1 << m.numQubits]);
+ComplexMatrixN bindArraysToStackComplexMatrixN(int numQubits, qreal re[][1 << numQubits], qreal im[][1 << numQubits],
+                                               qreal **reStorage, qreal **imStorage);
+#define UNPACK_ARR(...) __VA_ARGS__
+#define getStaticComplexMatrixN(numQubits, re, im)                                                             \
+    bindArraysToStackComplexMatrixN(numQubits, (qreal[1 << numQubits][1 << numQubits])UNPACK_ARR re,            \
+                                    (qreal[1 << numQubits][1 << numQubits])UNPACK_ARR im,                      \
+                                    (double *[1 << numQubits]){NULL}, (double *[1 << numQubits]){NULL})
+#endif
 
 /* ---- QASM recording (QuEST.h:3552-3591): accepted, records nothing -------- */
 void QUESTAPI startRecordingQASM(Qureg qureg);

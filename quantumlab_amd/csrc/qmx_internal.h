@@ -130,6 +130,32 @@ double densFidelity(const QuregDev *rho, const QuregDev *psi);     // <psi|rho|p
 void probOfAllOutcomes(const QuregDev *q, const int *qubits, int k, double *out);
 double normSquaredThenScale(QuregDev *q, bool rescale);            // FP64 norm, optional 1/sqrt renorm
 double maxAbsDiff(const QuregDev *a, const QuregDev *b);           // max_i |a_i - b_i|
+// <D> of a diagonal operator given as device FP64 arrays of 2^numQubitsRep entries
+Cplx expecDiagonal(const QuregDev *q, const double *opRe, const double *opIm);
+
+// ---- diagonal operators, phase functions (qmx_phase.hip) --------------------
+// amp_i *= D_i (state vector), rho_rc *= D_r (density matrix: rows only, as the
+// reference's densmatr_applyDiagonalOp)
+void applyDiagonal(QuregDev *q, const double *opRe, const double *opIm);
+// D_i = Σ_t c_t (-1)^parity(i & zMask_t) into device arrays of 2^numQubits entries
+void diagonalFromZHamil(double *opRe, double *opIm, int numQubits, const uint64_t *zMasks, const double *coeffs,
+                        int numTerms);
+// amp *= exp(i phase(x_1..x_R)) where x_r is the value of sub-register r (qubits
+// flattened, LSB first; encoding 0 unsigned, 1 two's complement). mode 0:
+// Σ_r Σ_t coeffs x_r^exponents; mode 1: the named function funcCode with params.
+// Overrides replace the phase at listed index tuples. conj negates the phase.
+struct PhaseFuncArgs {
+    int numRegs, encoding, mode, funcCode;
+    const int *qubits, *numQubitsPerReg;   // host arrays
+    const double *coeffs, *exponents;      // mode 0
+    const int *numTermsPerReg;             // mode 0
+    const double *params;                  // mode 1
+    int numParams;
+    const long long *overrideInds;         // numOverrides x numRegs
+    const double *overridePhases;
+    int numOverrides;
+};
+void applyPhaseFunction(QuregDev *q, const PhaseFuncArgs &args, int qubitShift, bool conj);
 
 // ---- non-unitary (qmx_density.hip) -----------------------------------------
 void collapse(QuregDev *q, int qubit, int outcome, double prob);

@@ -84,6 +84,29 @@ __global__ void __launch_bounds__(kBlock) kInner(const Amp<TB> *__restrict__ b, 
     writePartial<2>(acc, part);
 }
 
+// <D> for a diagonal operator D (FP64 op arrays): Σ |ψ_i|^2 D_i on a state vector,
+// Σ ρ_ii D_i on a density matrix (QuEST_cpu.c:4107-4209), Re and Im in one pass
+template <typename T, bool DENS>
+__global__ void __launch_bounds__(kBlock) kExpecDiag(const Amp<T> *__restrict__ a, long long dim,
+                                                     const double *__restrict__ opRe,
+                                                     const double *__restrict__ opIm, double *part) {
+    Acc acc[2];
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < dim; i += (long long)gridDim.x * kBlock) {
+        const double dr = opRe[i], di = opIm[i];
+        if (DENS) {
+            const Amp<T> v = a[i * (dim + 1)];
+            acc[0].add((double)v.x * dr - (double)v.y * di);
+            acc[1].add((double)v.x * di + (double)v.y * dr);
+        } else {
+            const Amp<T> v = a[i];
+            const double p = (double)v.x * v.x + (double)v.y * v.y;
+            acc[0].add(p * dr);
+            acc[1].add(p * di);
+        }
+    }
+    writePartial<2>(acc, part);
+}
+
 // density diagonal: Σ Re ρ_rr over rows r with bit q == 0 (or all rows)
 // QuEST_cpu.c:3385-3431, QuEST_cpu_local.c:121-142
 template <typename T>
@@ -244,6 +267,24 @@ double pairSum(const QuregDev *a, const QuregDev *b) {
 double probOfZero(const QuregDev *q, int qubit) { return sumSq(q, 1ull << qubit); }
 
 double totalProb(const QuregDev *q) { return sumSq(q, 0); }
+
+Cplx expecDiagonal(const QuregDev *q, const double *opRe, const double *opIm) {
+    ensureBuffers();
+    const long long dim = 1LL << q->numQubitsRep;
+    const unsigned grid = redGrid(dim);
+    QMX_DISPATCH(q->prec, T, {
+        if (q->isDensity)
+            hipLaunchKernelGGL((kExpecDiag<T, true>), dim3(grid), dim3(kBlock), 0, hstream(), (const Amp<T> *)q->amps,
+                               dim, opRe, opIm, g_partials);
+        else
+            hipLaunchKernelGGL((kExpecDiag<T, false>), dim3(grid), dim3(kBlock), 0, hstream(),
+                               (const Amp<T> *)q->amps, dim, opRe, opIm, g_partials);
+    });
+    checkLaunch("calcExpecDiagonalOp");
+    double r[2];
+    finish<2>(grid, r);
+    return Cplx{r[0], r[1]};
+}
 
 Cplx innerProduct(const QuregDev *bra, const QuregDev *ket) {
     ensureBuffers();

@@ -382,6 +382,60 @@ double expecPauliProd(const Qureg &q, const int *targs, const enum pauliOpType *
     return qmx::innerProduct(D(ws), D(q)).re;
 }
 
+// the phase-function passes of the reference: rows, then for a density matrix
+// the same function conjugated on the column bits (QuEST.c:732-870)
+void phaseFuncPasses(const Qureg &q, const qmx::PhaseFuncArgs &a) {
+    qmx::applyPhaseFunction(D(q), a, 0, false);
+    if (q.isDensityMatrix) qmx::applyPhaseFunction(D(q), a, q.numQubitsRepresented, true);
+}
+
+void hadamardRaw(const Qureg &q, int t) {
+    const double r = 1 / std::sqrt(2.0);
+    const Cplx H[4] = {{r, 0}, {r, 0}, {r, 0}, {-r, 0}};
+    gate2(q, t, 0, 0, H, H);
+}
+
+void swapRaw(const Qureg &q, int a, int b) {
+    if (q.isDensityMatrix) {
+        Cplx sw[16] = {};
+        sw[0] = sw[15] = Cplx{1, 0};
+        sw[1 * 4 + 2] = sw[2 * 4 + 1] = Cplx{1, 0};
+        int t[2] = {a, b};
+        qmx::densUnitary(D(q), t, 2, sw, sw);
+    } else {
+        qmx::applySwap(D(q), a, b);
+    }
+}
+
+// QuEST_common.c:846-908: from the top qubit down, H then the controlled
+// phases onto every lower qubit as one SCALED_PRODUCT phase function of
+// (lower qubits, this qubit) with coefficient pi / 2^q; then reverse the order
+void qftOn(const Qureg &q, const int *qubits, int n) {
+    for (int k = n - 1; k >= 0; --k) {
+        hadamardRaw(q, qubits[k]);
+        if (k == 0) break;
+        const int perReg[2] = {k, 1};
+        const double param = M_PI / (double)(1LL << k);
+        qmx::PhaseFuncArgs a{};
+        a.numRegs = 2;
+        a.encoding = UNSIGNED;
+        a.mode = 1;
+        a.funcCode = SCALED_PRODUCT;
+        a.qubits = qubits;
+        a.numQubitsPerReg = perReg;
+        a.params = &param;
+        a.numParams = 1;
+        phaseFuncPasses(q, a);
+    }
+    for (int i = 0; i < n / 2; ++i) swapRaw(q, qubits[i], qubits[n - 1 - i]);
+}
+
+DiagonalOp emptyDiagonalOp() {
+    DiagonalOp op;
+    std::memset(&op, 0, sizeof op);
+    return op;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1690,6 +1744,323 @@ void qmxSetErrorMode(int mode) { qv::setMode(mode); }
 int qmxLastError(char *buf, int len) { return qv::lastError(buf, len); }
 void qmxClearError(void) { qv::clear(); }
 int qmxDeviceInfo(char *buf, int len) { return qmx::deviceDescription(buf, len); }
+/* ---- diagonal operators (QuEST.c:1136-1146, 1237-1244, 1557-1606) ----------------------------- */
+
+DiagonalOp createDiagonalOp(int numQubits, QuESTEnv env) {
+    (void)env;
+    DiagonalOp op = emptyDiagonalOp();
+    API_TRY
+    qv::numQubitsInDiagOp(numQubits, __func__);
+    qmx::runtimeInit();
+    const long long n = numQubits < 62 ? 1LL << numQubits : 0;
+    op.numQubits = numQubits;
+    op.numElemsPerChunk = n;
+    op.numChunks = 1;
+    op.chunkId = 0;
+    op.real = n ? static_cast<qreal *>(std::calloc((size_t)n, sizeof(qreal))) : nullptr;
+    op.imag = n ? static_cast<qreal *>(std::calloc((size_t)n, sizeof(qreal))) : nullptr;
+    if (!op.real || !op.imag) {
+        std::free(op.real);
+        std::free(op.imag);
+        op = emptyDiagonalOp();
+        qv::raise(qv::message(qv::E_DIAGONAL_OP_NOT_ALLOCATED), __func__);
+    }
+    auto *dev = static_cast<qreal *>(qmxTryDevAlloc((size_t)n * 2 * sizeof(qreal)));
+    if (!dev) {
+        std::free(op.real);
+        std::free(op.imag);
+        op = emptyDiagonalOp();
+        qv::raise(qv::message(qv::E_DIAGONAL_OP_NOT_ALLOCATED_ON_GPU), __func__);
+    }
+    op.deviceOperator.real = dev;
+    op.deviceOperator.imag = dev + n;
+    qmx::hostToDevice(op.deviceOperator.real, op.real, (size_t)n * sizeof(qreal));
+    qmx::hostToDevice(op.deviceOperator.imag, op.imag, (size_t)n * sizeof(qreal));
+    return op;
+    API_CATCH(op)
+}
+
+void destroyDiagonalOp(DiagonalOp op, QuESTEnv env) {
+    (void)env;
+    API_TRY
+    qv::diagOpInit(op, __func__);
+    std::free(op.real);
+    std::free(op.imag);
+    qmx::devFree(op.deviceOperator.real);
+    API_CATCH_VOID
+}
+
+void syncDiagonalOp(DiagonalOp op) {
+    API_TRY
+    qv::diagOpInit(op, __func__);
+    const size_t bytes = (size_t)op.numElemsPerChunk * sizeof(qreal);
+    qmx::hostToDevice(op.deviceOperator.real, op.real, bytes);
+    qmx::hostToDevice(op.deviceOperator.imag, op.imag, bytes);
+    API_CATCH_VOID
+}
+
+// host and device copies both take the elements (QuEST_cpu.c:4211-4249 for the host side)
+static void setDiagElems(DiagonalOp op, long long start, const qreal *re, const qreal *im, long long n) {
+    std::memcpy(op.real + start, re, (size_t)n * sizeof(qreal));
+    std::memcpy(op.imag + start, im, (size_t)n * sizeof(qreal));
+    qmx::hostToDevice(op.deviceOperator.real + start, re, (size_t)n * sizeof(qreal));
+    qmx::hostToDevice(op.deviceOperator.imag + start, im, (size_t)n * sizeof(qreal));
+}
+
+void initDiagonalOp(DiagonalOp op, qreal *real, qreal *imag) {
+    API_TRY
+    qv::diagOpInit(op, __func__);
+    setDiagElems(op, 0, real, imag, 1LL << op.numQubits);
+    API_CATCH_VOID
+}
+
+void setDiagonalOpElems(DiagonalOp op, long long int startInd, qreal *real, qreal *imag, long long int numElems) {
+    API_TRY
+    qv::diagOpInit(op, __func__);
+    qv::numElems(op, startInd, numElems, __func__);
+    setDiagElems(op, startInd, real, imag, numElems);
+    API_CATCH_VOID
+}
+
+static void diagFromHamil(DiagonalOp op, const PauliHamil &h) {
+    std::vector<uint64_t> z(h.numSumTerms, 0);
+    for (int t = 0; t < h.numSumTerms; ++t)
+        for (int q = 0; q < h.numQubits; ++q)
+            if (h.pauliCodes[(size_t)t * h.numQubits + q] == PAULI_Z) z[t] |= 1ull << q;
+    std::vector<double> c(h.termCoeffs, h.termCoeffs + h.numSumTerms);
+    qmx::diagonalFromZHamil(op.deviceOperator.real, op.deviceOperator.imag, op.numQubits, z.data(), c.data(),
+                            h.numSumTerms);
+    const size_t bytes = (size_t)op.numElemsPerChunk * sizeof(qreal);
+    qmx::deviceToHost(op.real, op.deviceOperator.real, bytes);
+    qmx::deviceToHost(op.imag, op.deviceOperator.imag, bytes);
+}
+
+void initDiagonalOpFromPauliHamil(DiagonalOp op, PauliHamil hamil) {
+    API_TRY
+    qv::hamilParams(hamil.numQubits, hamil.numSumTerms, __func__);
+    qv::diagOpInit(op, __func__);
+    qv::diagPauliHamil(op, hamil, __func__);
+    diagFromHamil(op, hamil);
+    API_CATCH_VOID
+}
+
+DiagonalOp createDiagonalOpFromPauliHamilFile(char *fn, QuESTEnv env) {
+    DiagonalOp op = emptyDiagonalOp();
+    PauliHamil h = createPauliHamilFromFile(fn);  // validates the file; a recorded error leaves h empty
+    if (!h.pauliCodes) return op;
+    API_TRY
+    if (!qv::diagonalHamil(h)) {
+        destroyPauliHamil(h);
+        qv::raise(qv::message(qv::E_PAULI_HAMIL_NOT_DIAGONAL), __func__);
+    }
+    op = createDiagonalOp(h.numQubits, env);
+    if (op.real) diagFromHamil(op, h);
+    destroyPauliHamil(h);
+    return op;
+    API_CATCH(op)
+}
+
+void applyDiagonalOp(Qureg qureg, DiagonalOp op) {
+    API_TRY
+    qv::diagonalOp(qureg, op, __func__);
+    qmx::applyDiagonal(D(qureg), op.deviceOperator.real, op.deviceOperator.imag);
+    API_CATCH_VOID
+}
+
+Complex calcExpecDiagonalOp(Qureg qureg, DiagonalOp op) {
+    Complex z{0, 0};
+    API_TRY
+    qv::diagonalOp(qureg, op, __func__);
+    const Cplx r = qmx::expecDiagonal(D(qureg), op.deviceOperator.real, op.deviceOperator.imag);
+    z.real = r.re;
+    z.imag = r.im;
+    return z;
+    API_CATCH(z)
+}
+
+/* ---- phase functions and QFT (QuEST.c:732-891) -------------------------------------------------- */
+
+// each public entry validates under its own name (the reference's __func__)
+static void phaseFuncImpl(const Qureg &qureg, int *qubits, int numQubits, int encoding, qreal *coeffs,
+                          qreal *exponents, int numTerms, long long *overrideInds, qreal *overridePhases,
+                          int numOverrides, const char *caller) {
+    qv::phaseMultiQubits(qureg, qubits, numQubits, caller);
+    qv::bitEncoding(numQubits, encoding, caller);
+    if (overrideInds) qv::phaseFuncOverrides(numQubits, encoding, overrideInds, numOverrides, caller);
+    qv::phaseFuncTerms(numQubits, encoding, exponents, numTerms, overrideInds, numOverrides, caller);
+    qmx::PhaseFuncArgs a{};
+    a.numRegs = 1;
+    a.encoding = encoding;
+    a.mode = 0;
+    a.qubits = qubits;
+    a.numQubitsPerReg = &numQubits;
+    a.coeffs = coeffs;
+    a.exponents = exponents;
+    a.numTermsPerReg = &numTerms;
+    a.overrideInds = overrideInds;
+    a.overridePhases = overridePhases;
+    a.numOverrides = overrideInds ? numOverrides : 0;
+    phaseFuncPasses(qureg, a);
+}
+
+static void multiVarImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, int numRegs, int encoding,
+                         qreal *coeffs, qreal *exponents, int *numTermsPerReg, long long *overrideInds,
+                         qreal *overridePhases, int numOverrides, const char *caller) {
+    qv::qubitSubregs(qureg, qubits, numQubitsPerReg, numRegs, caller);
+    qv::multiRegBitEncoding(numQubitsPerReg, numRegs, encoding, caller);
+    qv::multiVarPhaseFuncTerms(numRegs, encoding, exponents, numTermsPerReg, caller);
+    if (overrideInds)
+        qv::multiVarPhaseFuncOverrides(numQubitsPerReg, numRegs, encoding, overrideInds, numOverrides, caller);
+    qmx::PhaseFuncArgs a{};
+    a.numRegs = numRegs;
+    a.encoding = encoding;
+    a.mode = 0;
+    a.qubits = qubits;
+    a.numQubitsPerReg = numQubitsPerReg;
+    a.coeffs = coeffs;
+    a.exponents = exponents;
+    a.numTermsPerReg = numTermsPerReg;
+    a.overrideInds = overrideInds;
+    a.overridePhases = overridePhases;
+    a.numOverrides = overrideInds ? numOverrides : 0;
+    phaseFuncPasses(qureg, a);
+}
+
+static void namedImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, int numRegs, int encoding, int code,
+                      qreal *params, int numParams, long long *overrideInds, qreal *overridePhases, int numOverrides,
+                      const char *caller) {
+    qv::qubitSubregs(qureg, qubits, numQubitsPerReg, numRegs, caller);
+    qv::multiRegBitEncoding(numQubitsPerReg, numRegs, encoding, caller);
+    qv::phaseFuncName(code, numRegs, numParams, caller);
+    if (overrideInds)
+        qv::multiVarPhaseFuncOverrides(numQubitsPerReg, numRegs, encoding, overrideInds, numOverrides, caller);
+    qmx::PhaseFuncArgs a{};
+    a.numRegs = numRegs;
+    a.encoding = encoding;
+    a.mode = 1;
+    a.funcCode = code;
+    a.qubits = qubits;
+    a.numQubitsPerReg = numQubitsPerReg;
+    a.params = params;
+    a.numParams = params ? numParams : 0;
+    a.overrideInds = overrideInds;
+    a.overridePhases = overridePhases;
+    a.numOverrides = overrideInds ? numOverrides : 0;
+    phaseFuncPasses(qureg, a);
+}
+
+void applyPhaseFunc(Qureg qureg, int *qubits, int numQubits, enum bitEncoding encoding, qreal *coeffs,
+                    qreal *exponents, int numTerms) {
+    API_TRY
+    phaseFuncImpl(qureg, qubits, numQubits, encoding, coeffs, exponents, numTerms, nullptr, nullptr, 0, __func__);
+    API_CATCH_VOID
+}
+
+void applyPhaseFuncOverrides(Qureg qureg, int *qubits, int numQubits, enum bitEncoding encoding, qreal *coeffs,
+                             qreal *exponents, int numTerms, long long int *overrideInds, qreal *overridePhases,
+                             int numOverrides) {
+    API_TRY
+    phaseFuncImpl(qureg, qubits, numQubits, encoding, coeffs, exponents, numTerms, overrideInds, overridePhases,
+                  numOverrides, __func__);
+    API_CATCH_VOID
+}
+
+void applyMultiVarPhaseFunc(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs, enum bitEncoding encoding,
+                            qreal *coeffs, qreal *exponents, int *numTermsPerReg) {
+    API_TRY
+    multiVarImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, coeffs, exponents, numTermsPerReg, nullptr,
+                 nullptr, 0, __func__);
+    API_CATCH_VOID
+}
+
+void applyMultiVarPhaseFuncOverrides(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs,
+                                     enum bitEncoding encoding, qreal *coeffs, qreal *exponents, int *numTermsPerReg,
+                                     long long int *overrideInds, qreal *overridePhases, int numOverrides) {
+    API_TRY
+    multiVarImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, coeffs, exponents, numTermsPerReg, overrideInds,
+                 overridePhases, numOverrides, __func__);
+    API_CATCH_VOID
+}
+
+void applyNamedPhaseFunc(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs, enum bitEncoding encoding,
+                         enum phaseFunc functionNameCode) {
+    API_TRY
+    namedImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, functionNameCode, nullptr, 0, nullptr, nullptr, 0,
+              __func__);
+    API_CATCH_VOID
+}
+
+void applyNamedPhaseFuncOverrides(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs,
+                                  enum bitEncoding encoding, enum phaseFunc functionNameCode,
+                                  long long int *overrideInds, qreal *overridePhases, int numOverrides) {
+    API_TRY
+    namedImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, functionNameCode, nullptr, 0, overrideInds,
+              overridePhases, numOverrides, __func__);
+    API_CATCH_VOID
+}
+
+void applyParamNamedPhaseFunc(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs, enum bitEncoding encoding,
+                              enum phaseFunc functionNameCode, qreal *params, int numParams) {
+    API_TRY
+    namedImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, functionNameCode, params, numParams, nullptr, nullptr,
+              0, __func__);
+    API_CATCH_VOID
+}
+
+void applyParamNamedPhaseFuncOverrides(Qureg qureg, int *qubits, int *numQubitsPerReg, int numRegs,
+                                       enum bitEncoding encoding, enum phaseFunc functionNameCode, qreal *params,
+                                       int numParams, long long int *overrideInds, qreal *overridePhases,
+                                       int numOverrides) {
+    API_TRY
+    namedImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, functionNameCode, params, numParams, overrideInds,
+              overridePhases, numOverrides, __func__);
+    API_CATCH_VOID
+}
+
+void applyQFT(Qureg qureg, int *qubits, int numQubits) {
+    API_TRY
+    qv::multiTargets(qureg, qubits, numQubits, __func__);
+    qftOn(qureg, qubits, numQubits);
+    API_CATCH_VOID
+}
+
+void applyFullQFT(Qureg qureg) {
+    API_TRY
+    std::vector<int> qs(qureg.numQubitsRepresented);
+    for (int i = 0; i < qureg.numQubitsRepresented; ++i) qs[i] = i;
+    qftOn(qureg, qs.data(), qureg.numQubitsRepresented);
+    API_CATCH_VOID
+}
+
+/* ---- C-only helpers for variable-length 2D arrays (QuEST.c:1425-1434, QuEST_common.c:649-664) ---
+ * Declared for C callers only (qreal re[][1 << n] decays to a pointer to rows of
+ * 2^n qreals: the same ABI as the flat row-major pointer taken here). */
+void initComplexMatrixN(ComplexMatrixN m, qreal *re, qreal *im) {
+    API_TRY
+    qv::matrixInit(m, __func__);
+    const int dim = 1 << m.numQubits;
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j) {
+            m.real[i][j] = re[(size_t)i * dim + j];
+            m.imag[i][j] = im[(size_t)i * dim + j];
+        }
+    API_CATCH_VOID
+}
+
+ComplexMatrixN bindArraysToStackComplexMatrixN(int numQubits, qreal *re, qreal *im, qreal **reStorage,
+                                               qreal **imStorage) {
+    ComplexMatrixN m;
+    m.numQubits = numQubits;
+    m.real = reStorage;
+    m.imag = imStorage;
+    const int dim = 1 << numQubits;
+    for (int i = 0; i < dim; ++i) {
+        m.real[i] = re + (size_t)i * dim;
+        m.imag[i] = im + (size_t)i * dim;
+    }
+    return m;
+}
+
 void qmxSync(void) { qmx::deviceSync(); }
 void qmxDensityFusionStats(long long *sweeps, long long *operations) { qmx::densSuperOpStats(sweeps, operations); }
 

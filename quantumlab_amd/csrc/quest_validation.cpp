@@ -15,6 +15,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 namespace qv {
 
@@ -101,6 +102,69 @@ const char *const kMessages[] = {
     /* E_MISMATCHING_PRECISION */
     "Registers must share the same storage precision for this operation.",
     /* E_INVALID_PRECISION */ "Invalid storage precision. Must be QMX_PREC_FP64 (0) or QMX_PREC_FP32 (1).",
+    /* E_INVALID_ELEM_INDEX */ "Invalid element index. Must be >=0 and <2^numQubits.",
+    /* E_INVALID_NUM_ELEMS */ "Invalid number of elements. Must be >=0 and <=2^numQubits.",
+    /* E_INVALID_OFFSET_NUM_ELEMS_DIAG */
+    "More elements given than exist in the diagonal operator from the given starting index.",
+    /* E_MISMATCHING_QUREG_DIAGONAL_OP_SIZE */
+    "The qureg must represent an equal number of qubits as that in the applied diagonal operator.",
+    /* E_DIAGONAL_OP_NOT_INITIALISED */
+    "The diagonal operator has not been initialised through createDiagonalOperator().",
+    /* E_PAULI_HAMIL_NOT_DIAGONAL */
+    "The Pauli Hamiltonian contained operators other than PAULI_Z and PAULI_I, and hence cannot be expressed as a "
+    "diagonal matrix.",
+    /* E_MISMATCHING_PAULI_HAMIL_DIAGONAL_OP_SIZE */
+    "The Pauli Hamiltonian and diagonal operator have different, incompatible dimensions.",
+    /* E_DIAGONAL_OP_NOT_ALLOCATED */ "Could not allocate memory for DiagonalOp. Possibly insufficient memory.",
+    /* E_DIAGONAL_OP_NOT_ALLOCATED_ON_GPU */
+    "Could not allocate memory for DiagonalOp on GPU. Possibly insufficient memory.",
+    /* E_INVALID_NUM_SUBREGISTERS */ "Invalid number of qubit subregisters, which must be >0 and <=100.",
+    /* E_INVALID_NUM_PHASE_FUNC_TERMS */ "Invalid number of terms in the phase function specified. Must be >0.",
+    /* E_INVALID_NUM_PHASE_FUNC_OVERRIDES */
+    "Invalid number of phase function overrides specified. Must be >=0, and for single-variable phase functions, "
+    "<=2^numQubits (the maximum unique binary values of the sub-register). Note that uniqueness of overriding "
+    "indices is not checked.",
+    /* E_INVALID_PHASE_FUNC_OVERRIDE_UNSIGNED_INDEX */
+    "Invalid phase function override index, in the UNSIGNED encoding. Must be >=0, and <= the maximum index "
+    "possible of the corresponding qubit subregister (2^numQubits-1).",
+    /* E_INVALID_PHASE_FUNC_OVERRIDE_TWOS_COMPLEMENT_INDEX */
+    "Invalid phase function override index, in the TWOS_COMPLEMENT encoding. Must be between (inclusive) "
+    "-2^(N-1) and +2^(N-1)-1, where N is the number of qubits (including the sign qubit).",
+    /* E_INVALID_PHASE_FUNC_NAME */
+    "Invalid named phase function, which must be one of {NORM, SCALED_NORM, INVERSE_NORM, SCALED_INVERSE_NORM, "
+    "PRODUCT, SCALED_PRODUCT, INVERSE_PRODUCT, SCALED_INVERSE_PRODUCT, DISTANCE, SCALED_DISTANCE, "
+    "INVERSE_DISTANCE, SCALED_INVERSE_DISTANCE}.",
+    /* E_INVALID_NUM_NAMED_PHASE_FUNC_PARAMS */
+    "Invalid number of parameters passed for the given named phase function. {NORM, PRODUCT, DISTANCE} accept 0 "
+    "parameters, {INVERSE_NORM, INVERSE_PRODUCT, INVERSE_DISTANCE} accept 1 parameter (the phase at the "
+    "divergence), {SCALED_NORM, SCALED_INVERSE_NORM, SCALED_PRODUCT} accept 1 parameter (the scaling "
+    "coefficient), {SCALED_INVERSE_PRODUCT, SCALED_DISTANCE, SCALED_INVERSE_DISTANCE} accept 2 parameters (the "
+    "coefficient then divergence phase), SCALED_INVERSE_SHIFTED_NORM accepts 2 + (number of sub-registers) "
+    "parameters (the coefficient, then the divergence phase, followed by the offset for each sub-register), "
+    "SCALED_INVERSE_SHIFTED_DISTANCE accepts 2 + (number of sub-registers) / 2 parameters (the coefficient, then "
+    "the divergence phase, followed by the offset for each pair of sub-registers).",
+    /* E_INVALID_NUM_REGS_DISTANCE_PHASE_FUNC */
+    "Phase functions DISTANCE, INVERSE_DISTANCE, SCALED_DISTANCE and SCALED_INVERSE_DISTANCE require a strictly "
+    "even number of sub-registers.",
+    /* E_INVALID_BIT_ENCODING */ "Invalid bit encoding. Must be one of {UNSIGNED, TWOS_COMPLEMENT}.",
+    /* E_INVALID_NUM_QUBITS_TWOS_COMPLEMENT */
+    "A sub-register contained too few qubits to employ TWOS_COMPLEMENT encoding. Must use >1 qubits (allocating "
+    "one for the sign).",
+    /* E_NEGATIVE_EXPONENT_WITHOUT_ZERO_OVERRIDE */
+    "The phase function contained a negative exponent which would diverge at zero, but the zero index was not "
+    "overriden.",
+    /* E_FRACTIONAL_EXPONENT_WITHOUT_NEG_OVERRIDE */
+    "The phase function contained a fractional exponent, which in TWOS_COMPLEMENT encoding, requires all negative "
+    "indices are overriden. However, one or more negative indices were not overriden.",
+    /* E_NEGATIVE_EXPONENT_MULTI_VAR */
+    "The phase function contained an illegal negative exponent. One must instead call applyPhaseFuncOverrides() "
+    "once for each register, so that the zero index of each register is overriden, independent of the indices of "
+    "all other registers.",
+    /* E_FRACTIONAL_EXPONENT_MULTI_VAR */
+    "The phase function contained a fractional exponent, which is illegal in TWOS_COMPLEMENT encoding, since it "
+    "cannot be (efficiently) checked that all negative indices were overriden. One must instead call "
+    "applyPhaseFuncOverrides() once for each register, so that each register's negative indices can be "
+    "overriden, independent of the indices of all other registers.",
 };
 static_assert(sizeof(kMessages) / sizeof(kMessages[0]) == E_NUM_CODES, "message table out of sync");
 
@@ -426,6 +490,158 @@ void matchingPrecision(const Qureg &a, const Qureg &b, const char *f) {
     check(qmx::dev(a)->prec == qmx::dev(b)->prec, E_MISMATCHING_PRECISION, f);
 }
 void precision(int p, const char *f) { check(p == 0 || p == 1, E_INVALID_PRECISION, f); }
+
+
+// ---- diagonal operators ----------------------------------------------------------
+void numQubitsInDiagOp(int numQubits, const char *f) {
+    check(numQubits > 0, E_INVALID_NUM_CREATE_QUBITS, f);
+    check(numQubits <= 63, E_NUM_AMPS_EXCEED_TYPE, f);
+}
+void numElems(const DiagonalOp &op, long long start, long long n, const char *f) {
+    const long long mx = 1LL << op.numQubits;
+    check(start >= 0 && start < mx, E_INVALID_ELEM_INDEX, f);
+    check(n >= 0 && n <= mx, E_INVALID_NUM_ELEMS, f);
+    check(n + start <= mx, E_INVALID_OFFSET_NUM_ELEMS_DIAG, f);
+}
+void diagOpInit(const DiagonalOp &op, const char *f) {
+    check(op.real != nullptr && op.imag != nullptr, E_DIAGONAL_OP_NOT_INITIALISED, f);
+}
+void diagonalOp(const Qureg &q, const DiagonalOp &op, const char *f) {
+    diagOpInit(op, f);
+    check(q.numQubitsRepresented == op.numQubits, E_MISMATCHING_QUREG_DIAGONAL_OP_SIZE, f);
+}
+bool diagonalHamil(const PauliHamil &h) {
+    for (long long p = 0; p < (long long)h.numSumTerms * h.numQubits; ++p)
+        if (h.pauliCodes[p] != PAULI_I && h.pauliCodes[p] != PAULI_Z) return false;
+    return true;
+}
+void diagPauliHamil(const DiagonalOp &op, const PauliHamil &h, const char *f) {
+    check(op.numQubits == h.numQubits, E_MISMATCHING_PAULI_HAMIL_DIAGONAL_OP_SIZE, f);
+    check(diagonalHamil(h), E_PAULI_HAMIL_NOT_DIAGONAL, f);
+}
+
+// ---- phase functions -----------------------------------------------------------------
+namespace {
+constexpr int kMaxRegs = 100;  // MAX_NUM_REGS_APPLY_ARBITRARY_PHASE (QuEST_precision.h)
+bool isFraction(double e) { return std::floor(e) != e; }
+}  // namespace
+
+void phaseMultiQubits(const Qureg &q, const int *qs, int n, const char *f) {
+    check(n > 0 && n <= q.numQubitsRepresented, E_INVALID_NUM_QUBITS, f);
+    for (int i = 0; i < n; ++i) check(qs[i] >= 0 && qs[i] < q.numQubitsRepresented, E_INVALID_QUBIT_INDEX, f);
+    check(uniqueQubits(qs, n), E_QUBITS_NOT_UNIQUE, f);
+}
+void qubitSubregs(const Qureg &q, const int *qubits, const int *numQubitsPerReg, int numRegs, const char *f) {
+    check(numRegs > 0 && numRegs <= kMaxRegs, E_INVALID_NUM_SUBREGISTERS, f);
+    int i = 0;
+    for (int r = 0; r < numRegs; ++r) {
+        check(numQubitsPerReg[r] > 0 && numQubitsPerReg[r] <= q.numQubitsRepresented, E_INVALID_NUM_QUBITS, f);
+        for (int k = 0; k < numQubitsPerReg[r]; ++k, ++i)
+            check(qubits[i] >= 0 && qubits[i] < q.numQubitsRepresented, E_INVALID_QUBIT_INDEX, f);
+    }
+    check(uniqueQubits(qubits, i), E_QUBITS_NOT_UNIQUE, f);
+}
+void bitEncoding(int numQubits, int encoding, const char *f) {
+    check(encoding == UNSIGNED || encoding == TWOS_COMPLEMENT, E_INVALID_BIT_ENCODING, f);
+    if (encoding == TWOS_COMPLEMENT) check(numQubits > 1, E_INVALID_NUM_QUBITS_TWOS_COMPLEMENT, f);
+}
+void multiRegBitEncoding(const int *numQubitsPerReg, int numRegs, int encoding, const char *f) {
+    check(encoding == UNSIGNED || encoding == TWOS_COMPLEMENT, E_INVALID_BIT_ENCODING, f);
+    if (encoding == TWOS_COMPLEMENT)
+        for (int r = 0; r < numRegs; ++r) check(numQubitsPerReg[r] > 1, E_INVALID_NUM_QUBITS_TWOS_COMPLEMENT, f);
+}
+void phaseFuncTerms(int numQubits, int encoding, const double *exponents, int numTerms,
+                    const long long *overrideInds, int numOverrides, const char *f) {
+    check(numTerms > 0, E_INVALID_NUM_PHASE_FUNC_TERMS, f);
+    bool frac = false, neg = false;
+    for (int t = 0; t < numTerms; ++t) {
+        frac = frac || isFraction(exponents[t]);
+        neg = neg || exponents[t] < 0;
+    }
+    // a negative power diverges at index 0, which must then be overridden
+    if (neg) {
+        bool zero = false;
+        for (int v = 0; v < numOverrides && !zero; ++v) zero = overrideInds[v] == 0;
+        check(zero, E_NEGATIVE_EXPONENT_WITHOUT_ZERO_OVERRIDE, f);
+    }
+    // a fractional power of a negative index is undefined: every negative index
+    // must be overridden (checked exactly below 16 qubits, trusted above)
+    if (frac && encoding == TWOS_COMPLEMENT) {
+        const long long numNeg = 1LL << (numQubits - 1);
+        check(numOverrides >= numNeg, E_FRACTIONAL_EXPONENT_WITHOUT_NEG_OVERRIDE, f);
+        if (numQubits < 16) {
+            std::vector<char> seen((size_t)numNeg, 0);
+            for (int v = 0; v < numOverrides; ++v)
+                if (overrideInds[v] < 0 && -1 - overrideInds[v] < numNeg) seen[(size_t)(-1 - overrideInds[v])] = 1;
+            bool all = true;
+            for (long long i = 0; i < numNeg && all; ++i) all = seen[(size_t)i] != 0;
+            check(all, E_FRACTIONAL_EXPONENT_WITHOUT_NEG_OVERRIDE, f);
+        }
+    }
+}
+void multiVarPhaseFuncTerms(int numRegs, int encoding, const double *exponents, const int *numTermsPerReg,
+                            const char *f) {
+    check(numRegs > 0 && numRegs <= kMaxRegs, E_INVALID_NUM_SUBREGISTERS, f);
+    int total = 0;
+    for (int r = 0; r < numRegs; ++r) {
+        check(numTermsPerReg[r] > 0, E_INVALID_NUM_PHASE_FUNC_TERMS, f);
+        total += numTermsPerReg[r];
+    }
+    bool frac = false, neg = false;
+    for (int t = 0; t < total; ++t) {
+        frac = frac || isFraction(exponents[t]);
+        neg = neg || exponents[t] < 0;
+    }
+    check(!neg, E_NEGATIVE_EXPONENT_MULTI_VAR, f);
+    if (encoding == TWOS_COMPLEMENT) check(!frac, E_FRACTIONAL_EXPONENT_MULTI_VAR, f);
+}
+void phaseFuncOverrides(int numQubits, int encoding, const long long *overrideInds, int numOverrides,
+                        const char *f) {
+    check(numOverrides >= 0, E_INVALID_NUM_PHASE_FUNC_OVERRIDES, f);
+    check(numOverrides <= (1 << numQubits), E_INVALID_NUM_PHASE_FUNC_OVERRIDES, f);
+    if (encoding == UNSIGNED) {
+        const long long mx = (1LL << numQubits) - 1;
+        for (int v = 0; v < numOverrides; ++v)
+            check(overrideInds[v] >= 0 && overrideInds[v] <= mx, E_INVALID_PHASE_FUNC_OVERRIDE_UNSIGNED_INDEX, f);
+    } else if (encoding == TWOS_COMPLEMENT) {
+        const long long mn = -(1LL << (numQubits - 1)), mx = (1LL << (numQubits - 1)) - 1;
+        for (int v = 0; v < numOverrides; ++v)
+            check(overrideInds[v] >= mn && overrideInds[v] <= mx, E_INVALID_PHASE_FUNC_OVERRIDE_TWOS_COMPLEMENT_INDEX,
+                  f);
+    }
+}
+void multiVarPhaseFuncOverrides(const int *numQubitsPerReg, int numRegs, int encoding,
+                                const long long *overrideInds, int numOverrides, const char *f) {
+    check(numOverrides >= 0, E_INVALID_NUM_PHASE_FUNC_OVERRIDES, f);
+    int i = 0;
+    for (int v = 0; v < numOverrides; ++v)
+        for (int r = 0; r < numRegs; ++r, ++i) {
+            if (encoding == UNSIGNED) {
+                const long long mx = (1LL << numQubitsPerReg[r]) - 1;
+                check(overrideInds[i] >= 0 && overrideInds[i] <= mx, E_INVALID_PHASE_FUNC_OVERRIDE_UNSIGNED_INDEX, f);
+            } else if (encoding == TWOS_COMPLEMENT) {
+                const long long mn = -(1LL << (numQubitsPerReg[r] - 1)), mx = (1LL << (numQubitsPerReg[r] - 1)) - 1;
+                check(overrideInds[i] >= mn && overrideInds[i] <= mx,
+                      E_INVALID_PHASE_FUNC_OVERRIDE_TWOS_COMPLEMENT_INDEX, f);
+            }
+        }
+}
+void phaseFuncName(int c, int numRegs, int numParams, const char *f) {
+    check(c >= NORM && c <= SCALED_INVERSE_SHIFTED_DISTANCE, E_INVALID_PHASE_FUNC_NAME, f);
+    int want = 0;
+    switch (c) {
+    case NORM: case PRODUCT: case DISTANCE: want = 0; break;
+    case INVERSE_NORM: case INVERSE_PRODUCT: case INVERSE_DISTANCE:
+    case SCALED_NORM: case SCALED_PRODUCT: case SCALED_DISTANCE: want = 1; break;
+    case SCALED_INVERSE_NORM: case SCALED_INVERSE_PRODUCT: case SCALED_INVERSE_DISTANCE: want = 2; break;
+    case SCALED_INVERSE_SHIFTED_NORM: want = 2 + numRegs; break;
+    case SCALED_INVERSE_SHIFTED_DISTANCE: want = 2 + numRegs / 2; break;
+    }
+    check(numParams == want, E_INVALID_NUM_NAMED_PHASE_FUNC_PARAMS, f);
+    if (c == DISTANCE || c == INVERSE_DISTANCE || c == SCALED_DISTANCE || c == SCALED_INVERSE_DISTANCE ||
+        c == SCALED_INVERSE_SHIFTED_DISTANCE)
+        check(numRegs % 2 == 0, E_INVALID_NUM_REGS_DISTANCE_PHASE_FUNC, f);
+}
 
 }  // namespace qv
 

@@ -72,6 +72,29 @@ enum ErrorCode {
     E_MATRIX_TOO_LARGE_FOR_DEVICE,
     E_MISMATCHING_PRECISION,
     E_INVALID_PRECISION,
+    E_INVALID_ELEM_INDEX,
+    E_INVALID_NUM_ELEMS,
+    E_INVALID_OFFSET_NUM_ELEMS_DIAG,
+    E_MISMATCHING_QUREG_DIAGONAL_OP_SIZE,
+    E_DIAGONAL_OP_NOT_INITIALISED,
+    E_PAULI_HAMIL_NOT_DIAGONAL,
+    E_MISMATCHING_PAULI_HAMIL_DIAGONAL_OP_SIZE,
+    E_DIAGONAL_OP_NOT_ALLOCATED,
+    E_DIAGONAL_OP_NOT_ALLOCATED_ON_GPU,
+    E_INVALID_NUM_SUBREGISTERS,
+    E_INVALID_NUM_PHASE_FUNC_TERMS,
+    E_INVALID_NUM_PHASE_FUNC_OVERRIDES,
+    E_INVALID_PHASE_FUNC_OVERRIDE_UNSIGNED_INDEX,
+    E_INVALID_PHASE_FUNC_OVERRIDE_TWOS_COMPLEMENT_INDEX,
+    E_INVALID_PHASE_FUNC_NAME,
+    E_INVALID_NUM_NAMED_PHASE_FUNC_PARAMS,
+    E_INVALID_NUM_REGS_DISTANCE_PHASE_FUNC,
+    E_INVALID_BIT_ENCODING,
+    E_INVALID_NUM_QUBITS_TWOS_COMPLEMENT,
+    E_NEGATIVE_EXPONENT_WITHOUT_ZERO_OVERRIDE,
+    E_FRACTIONAL_EXPONENT_WITHOUT_NEG_OVERRIDE,
+    E_NEGATIVE_EXPONENT_MULTI_VAR,
+    E_FRACTIONAL_EXPONENT_MULTI_VAR,
     E_NUM_CODES
 };
 
@@ -138,6 +161,28 @@ void hamilParams(int numQubits, int numTerms, const char *f);
 void pauliHamil(const PauliHamil &h, const char *f);
 void matchingQuregPauliHamilDims(const Qureg &q, const PauliHamil &h, const char *f);
 void trotterParams(int order, int reps, const char *f);
+
+// diagonal operators (QuEST_validation.c:399-409, 436-441, 773-819)
+void numQubitsInDiagOp(int numQubits, const char *f);
+void numElems(const DiagonalOp &op, long long start, long long n, const char *f);
+void diagOpInit(const DiagonalOp &op, const char *f);
+void diagonalOp(const Qureg &q, const DiagonalOp &op, const char *f);
+void diagPauliHamil(const DiagonalOp &op, const PauliHamil &h, const char *f);
+bool diagonalHamil(const PauliHamil &h);   // only PAULI_I and PAULI_Z codes
+// phase functions (QuEST_validation.c:487-493, 821-1050)
+void phaseMultiQubits(const Qureg &q, const int *qs, int n, const char *f);
+void qubitSubregs(const Qureg &q, const int *qubits, const int *numQubitsPerReg, int numRegs, const char *f);
+void bitEncoding(int numQubits, int encoding, const char *f);
+void multiRegBitEncoding(const int *numQubitsPerReg, int numRegs, int encoding, const char *f);
+void phaseFuncTerms(int numQubits, int encoding, const double *exponents, int numTerms,
+                    const long long *overrideInds, int numOverrides, const char *f);
+void multiVarPhaseFuncTerms(int numRegs, int encoding, const double *exponents, const int *numTermsPerReg,
+                            const char *f);
+void phaseFuncOverrides(int numQubits, int encoding, const long long *overrideInds, int numOverrides,
+                        const char *f);
+void multiVarPhaseFuncOverrides(const int *numQubitsPerReg, int numRegs, int encoding,
+                                const long long *overrideInds, int numOverrides, const char *f);
+void phaseFuncName(int funcCode, int numRegs, int numParams, const char *f);
 void matchingPrecision(const Qureg &a, const Qureg &b, const char *f);
 void precision(int p, const char *f);
 

@@ -31,6 +31,12 @@ class QASMLogger(C.Structure):
                 ("isLogging", C.c_int)]
 
 
+class DiagonalOp(C.Structure):
+    _fields_ = [("numQubits", C.c_int), ("numElemsPerChunk", C.c_longlong), ("numChunks", C.c_int),
+                ("chunkId", C.c_int), ("real", C.POINTER(qreal)), ("imag", C.POINTER(qreal)),
+                ("deviceOperator", ComplexArray)]
+
+
 class Qureg(C.Structure):
     _fields_ = [
         ("isDensityMatrix", C.c_int),
@@ -174,6 +180,35 @@ _SIGS = {
     "multiRotateZ": (None, [Qureg, C.POINTER(C.c_int), C.c_int, qreal]),
     "multiRotatePauli": (None, [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, qreal]),
     "applyTrotterCircuit": (None, [Qureg, PauliHamil, qreal, C.c_int, C.c_int]),
+    "createDiagonalOp": (DiagonalOp, [C.c_int, QuESTEnv]),
+    "destroyDiagonalOp": (None, [DiagonalOp, QuESTEnv]),
+    "syncDiagonalOp": (None, [DiagonalOp]),
+    "initDiagonalOp": (None, [DiagonalOp, C.POINTER(qreal), C.POINTER(qreal)]),
+    "setDiagonalOpElems": (None, [DiagonalOp, C.c_longlong, C.POINTER(qreal), C.POINTER(qreal), C.c_longlong]),
+    "initDiagonalOpFromPauliHamil": (None, [DiagonalOp, PauliHamil]),
+    "createDiagonalOpFromPauliHamilFile": (DiagonalOp, [C.c_char_p, QuESTEnv]),
+    "applyDiagonalOp": (None, [Qureg, DiagonalOp]),
+    "calcExpecDiagonalOp": (Complex, [Qureg, DiagonalOp]),
+    "applyPhaseFunc": (None, [Qureg, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(qreal), C.POINTER(qreal),
+                              C.c_int]),
+    "applyPhaseFuncOverrides": (None, [Qureg, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(qreal),
+                                       C.POINTER(qreal), C.c_int, C.POINTER(C.c_longlong), C.POINTER(qreal),
+                                       C.c_int]),
+    "applyMultiVarPhaseFunc": (None, [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int,
+                                      C.POINTER(qreal), C.POINTER(qreal), C.POINTER(C.c_int)]),
+    "applyMultiVarPhaseFuncOverrides": (None, [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int,
+                                               C.POINTER(qreal), C.POINTER(qreal), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_longlong), C.POINTER(qreal), C.c_int]),
+    "applyNamedPhaseFunc": (None, [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int]),
+    "applyNamedPhaseFuncOverrides": (None, [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int,
+                                            C.c_int, C.POINTER(C.c_longlong), C.POINTER(qreal), C.c_int]),
+    "applyParamNamedPhaseFunc": (None, [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int,
+                                        C.POINTER(qreal), C.c_int]),
+    "applyParamNamedPhaseFuncOverrides": (None, [Qureg, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.c_int,
+                                                 C.c_int, C.POINTER(qreal), C.c_int, C.POINTER(C.c_longlong),
+                                                 C.POINTER(qreal), C.c_int]),
+    "applyQFT": (None, [Qureg, C.POINTER(C.c_int), C.c_int]),
+    "applyFullQFT": (None, [Qureg]),
     "calcTotalProb": (qreal, [Qureg]),
     "calcProbOfOutcome": (qreal, [Qureg, C.c_int, C.c_int]),
     "calcProbOfAllOutcomes": (None, [C.POINTER(qreal), Qureg, C.POINTER(C.c_int), C.c_int]),
