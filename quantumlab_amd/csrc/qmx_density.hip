@@ -60,43 +60,6 @@ __global__ void __launch_bounds__(kBlock) kDensCollapse(Amp<T> *__restrict__ a, 
     }
 }
 
-// Dense superoperator on the 4^K entries of rho that share every bit outside
-// the K targets' row and column bits: one thread per group, entries held in
-// registers, the matrix read through scalar loads (uniform across the wave).
-struct SuperSpec {
-    int fix[4];          // ascending row/column bit positions of the targets
-    uint64_t off[16];    // address offset of group entry e
-};
-
-template <typename T, int K, bool NT>
-__global__ void __launch_bounds__(kBlock) kSuperOp(Amp<T> *__restrict__ a, long long groups, SuperSpec sp,
-                                                   const Amp<T> *__restrict__ S) {
-    constexpr int E = 1 << (2 * K);
-    const long long g = xcdBlock() * kBlock + threadIdx.x;
-    if (g >= groups) return;
-    uint64_t base = (uint64_t)g;
-#pragma unroll
-    for (int j = 0; j < 2 * K; ++j) base = insertZero(base, sp.fix[j]);
-    T xr[E], xi[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        Amp<T> v = ldA<T, NT>(a + (base | sp.off[e]));
-        xr[e] = v.x;
-        xi[e] = v.y;
-    }
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-        T yr = 0, yi = 0;
-#pragma unroll
-        for (int c = 0; c < E; ++c) {
-            const Amp<T> m = S[r * E + c];
-            yr = fma(m.x, xr[c], fma(-m.y, xi[c], yr));
-            yi = fma(m.x, xi[c], fma(m.y, xr[c], yi));
-        }
-        stA<T, NT>(a + (base | sp.off[r]), Amp<T>{yr, yi});
-    }
-}
-
 // two-qubit dephasing (QuEST_cpu.c:97-136): scale where either qubit's row and
 // column bits differ
 template <typename T>
@@ -149,8 +112,6 @@ namespace {
 
 constexpr int kPendMax = 2;                // qubits one pending superoperator spans
 constexpr int kMaxEntries = 256;           // (4^2)^2
-constexpr int kRing = 256;                 // matrix slots in flight
-constexpr size_t kSlotBytes = kMaxEntries * sizeof(Amp<double>);
 
 // Pending work of one register: superoperators on disjoint qubit sets of at
 // most two qubits each, in creation order. Blocks on disjoint qubits commute,
@@ -167,75 +128,14 @@ struct Pending {
 Pending g_pend;
 long long g_passes = 0, g_absorbed = 0;
 
-// pinned staging ring for the matrices: slot i is rewritten only after the copy
-// that last read it has completed; copies and launches share the stream, so
-// the device slot is reused only after the kernel that read it has finished
-struct MatrixRing {
-    char *dev = nullptr, *host = nullptr;
-    hipEvent_t ev[kRing];
-    bool used[kRing] = {};
-    int next = 0;
-};
-MatrixRing g_ring;
-
-template <typename T>
-const Amp<T> *stageMatrix(const Cplx *S, int entries, hipStream_t st) {
-    if (!g_ring.dev) {
-        g_ring.dev = static_cast<char *>(devAlloc(kRing * kSlotBytes));
-        QMX_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&g_ring.host), kRing * kSlotBytes,
-                                    hipHostMallocDefault));
-        for (int i = 0; i < kRing; ++i)
-            QMX_HIP_CHECK(hipEventCreateWithFlags(&g_ring.ev[i], hipEventDisableTiming));
-    }
-    const int slot = g_ring.next;
-    g_ring.next = (slot + 1) % kRing;
-    if (g_ring.used[slot]) QMX_HIP_CHECK(hipEventSynchronize(g_ring.ev[slot]));
-    Amp<T> *h = reinterpret_cast<Amp<T> *>(g_ring.host + slot * kSlotBytes);
-    for (int e = 0; e < entries; ++e) h[e] = Amp<T>{(T)S[e].re, (T)S[e].im};
-    char *d = g_ring.dev + slot * kSlotBytes;
-    QMX_HIP_CHECK(hipMemcpyAsync(d, h, entries * sizeof(Amp<T>), hipMemcpyHostToDevice, st));
-    QMX_HIP_CHECK(hipEventRecord(g_ring.ev[slot], st));
-    g_ring.used[slot] = true;
-    return reinterpret_cast<const Amp<T> *>(d);
-}
-
-template <typename T, int K>
-void launchSuperOpT(QuregDev *q, long long groups, const SuperSpec &sp, bool nt, const Cplx *S,
-                    hipStream_t st) {
-    const Amp<T> *m = stageMatrix<T>(S, 1 << (4 * K), st);
-    const unsigned grid = gridFor(groups, 1);
-    if (nt)
-        hipLaunchKernelGGL((kSuperOp<T, K, true>), dim3(grid), dim3(kBlock), 0, st, (Amp<T> *)q->amps, groups,
-                           sp, m);
-    else
-        hipLaunchKernelGGL((kSuperOp<T, K, false>), dim3(grid), dim3(kBlock), 0, st, (Amp<T> *)q->amps,
-                           groups, sp, m);
-}
-
 void launchSuperOp(QuregDev *q, const int *t, int k, const Cplx *S) {
-    const int n = q->numQubitsRep;
-    int pos[2 * kPendMax];
+    // index bits 0..k-1: the targets' row bits; k..2k-1: their column bits
+    int bits[2 * kPendMax];
     for (int i = 0; i < k; ++i) {
-        pos[i] = t[i];
-        pos[k + i] = t[i] + n;
+        bits[i] = t[i];
+        bits[k + i] = t[i] + q->numQubitsRep;
     }
-    SuperSpec sp{};
-    for (int e = 0; e < (1 << (2 * k)); ++e) {
-        uint64_t off = 0;
-        for (int j = 0; j < 2 * k; ++j)
-            if ((e >> j) & 1) off |= 1ull << pos[j];
-        sp.off[e] = off;
-    }
-    std::sort(pos, pos + 2 * k);
-    for (int j = 0; j < 2 * k; ++j) sp.fix[j] = pos[j];
-    const long long groups = q->numAmps >> (2 * k);
-    const bool nt = useNT(sp.fix[0], q->prec);
-    hipStream_t st = hstream();
-    QMX_DISPATCH(q->prec, T, {
-        if (k == 1) launchSuperOpT<T, 1>(q, groups, sp, nt, S, st);
-        else launchSuperOpT<T, 2>(q, groups, sp, nt, S, st);
-    });
-    checkLaunch("density superoperator");
+    applyDense(q, bits, 2 * k, S);
 }
 
 void sweepBlock(QuregDev *q, const Block &b) {

@@ -341,6 +341,11 @@ void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const
         fused5Free(p);
         return;
     }
+    if (ctrlMask == 0 && denseBitsSupported(k) && n >= k) {
+        // uncontrolled 2..5-target matrices: one group per thread in registers
+        applyDense(q, targs, k, u);
+        return;
+    }
     const int c = __builtin_popcountll(ctrlMask);
     uint64_t tmask = 0;
     for (int j = 0; j < k; ++j) tmask |= 1ull << targs[j];

@@ -117,6 +117,11 @@ int pauliSequenceBlocks(int numQubits, int prec, const PauliExp *seq, int count)
 // |0...0> of the register (qmx_swaptest.hip)
 void amplitudeEncode(QuregDev *q, const double *vec, int qbits);
 
+// dense 2^B x 2^B matrix (row-major; index bit j <-> register bit bits[j]) on
+// B = 2..5 arbitrary bits, one group per thread in registers (qmx_dense.hip)
+bool denseBitsSupported(int B);
+void applyDense(QuregDev *q, const int *bits, int B, const Cplx *M);
+
 // ---- reductions (qmx_reduce.hip); all accumulate in FP64 -------------------
 double probOfZero(const QuregDev *q, int qubit);                   // state vector
 double totalProb(const QuregDev *q);                               // state vector
