@@ -63,8 +63,10 @@ def main():
             f.write(f"# {args.name}: kernel durations (rocprofv3 kernel trace)\n\n")
             f.write(trace_table(args.trace, args.trace_cmd) + "\n")
         return
+    # the state-vector legs only (per-gate FP64 and fused FP32), so the sweep
+    # kernels' averages are those of the 32-qubit registers
     trace_cmd = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_trace -o bench -- "
-                 "python3 bench.py --steps 1 --warmup 0 --no-cpu")
+                 "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-swap --no-density --no-trotter")
     pmc_cmd = ("rocprofv3 --pmc {c} --output-format csv -d gpurun_out/prof_{d} -o bench -- "
                "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-fused")
     fetch, calls = counter_means(args.fetch)
