@@ -300,12 +300,13 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
             "success_rate": float(np.mean(inf <= eps)), "max_infidelity_vs_4x_finer_fp64": float(inf.max()),
             "step_doubling_met_eps": bool(all(x[5][2][2] and x[5][4][2] for x in rows)),
             "pauli_exponentials_per_s_learned": round(exps / tl.sum(), 1),
-            "roofline": {"bound": "hbm", "kernel": "kPauliBlock<float> (fused Pauli-rotation sweeps)",
+            "roofline": {"bound": "hbm", "kernel": "kPauliCoset<float> (fused Pauli-rotation sweeps)",
                          "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                          "sweeps": int(sum(x[6] for x in rows)), "bytes_per_sweep": 2.0 * (1 << n) * 8,
                          "note": "read + write of the FP32 register per fused block, over the learned solves' wall "
-                                 "time; tile load latency and the per-pass LDS round trips hold it below HBM"},
+                                 "time, host planning and launch gaps included; one LDS round trip per coset group "
+                                 "(about 1.7 per block) on top of the tile's HBM read + write"},
             "choices": [[x[1], x[2]] for x in rows],
             "step_doubling_reps": [[x[5][2][1], x[5][4][1]] for x in rows]}, rows
 

@@ -14,6 +14,9 @@
 // rotations run in passes: consecutive rotations whose X/Y masks coincide
 // (e.g. XX and YY on one bond), plus any diagonal rotations between them, are
 // applied to each thread's amplitude pairs in registers in one LDS round trip.
+// Full tiles use the coset form instead (kPauliCoset): one LDS round trip per
+// group of rotations whose X/Y masks span at most 4 (FP32) / 3 (FP64)
+// dimensions, each thread holding one coset of that span in registers.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -222,15 +225,278 @@ __global__ void __launch_bounds__(kBlock) kPauliBlock(Amp<T> *__restrict__ a, un
     }
 }
 
+// ---- coset form: a thread holds a coset of the exchange masks' span ----------
+// Full tiles only (2^(8 + R) amplitudes, R = 4 for FP32, 3 for FP64). A block's
+// rotations split into groups of consecutive rotations whose tile-local X/Y
+// masks span at most R dimensions over GF(2). For a group, the host puts a
+// basis b_0..b_{R-1} of a span containing them in reduced echelon form (pivot
+// bit piv_j set in b_j only) and thread t takes the coset r + span, where r =
+// t with zeros inserted at the pivots: element v (bit j of v selects b_j) is
+// tile index r ^ C[v]. Every pair (i, i ^ lx) of every rotation in the group is
+// then a pair (v, v ^ m) of one thread's registers, m = lx's coordinates, so
+// the group costs one LDS round trip where the pass form costs one per
+// distinct mask. Signs: parity(index & lz) = parity(r & lz) xor parity(v & zv)
+// with zv_j = parity(b_j & lz).
+struct CosetExpDev {
+    uint64_t z;          // global z mask (parity of the tile base)
+    unsigned lz;         // tile-local z mask
+    unsigned m;          // the X/Y mask's coordinates in the group's basis (0: diagonal)
+    unsigned zv;         // bit j: parity(b_j & lz)
+    int kind;            // as PauliExpDev
+    double c, kap;
+};
+
+constexpr int kMaxCosetBits = 4;
+
+struct CosetGroupDev {
+    int first, count;             // the group's rotations (block-relative)
+    int piv[kMaxCosetBits];       // pivot tile bits, ascending
+    unsigned C[1 << kMaxCosetBits];  // tile offset of coset element v
+};
+
+template <typename T> constexpr int cosetBits() { return sizeof(T) == 8 ? 3 : 4; }
+
+// A thread's coset, real and imaginary parts as vector values: they stay SSA
+// values through the exchange-mask switch below (an array there is kept in
+// scratch memory once the compiler merges the cases' stores).
+template <typename T, int E> struct CosetRegs {
+    typedef T V __attribute__((ext_vector_type(E)));
+    V re, im;
+};
+
+// out_u = c x_u + kap s(u ^ M) F x_{u ^ M} for every u, F = 1 (KIND 1, 4) or i
+// (KIND 2, 5), s(v) = (-1)^parity(v & zv) (the thread's own parity is in kap);
+// KIND 4, 5: the tangent form (c = 1)
+template <typename T, int E, int M, int KIND>
+__device__ __forceinline__ void cosetPairs(CosetRegs<T, E> &x, T c, T kap, unsigned zv) {
+    if constexpr (M < E) {
+        constexpr int lowM = M & -M;
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            if (u & lowM) continue;
+            const int p = u ^ M;
+            const T Bu = signFlip(kap, (unsigned)__popc((unsigned)p & zv));  // for out_u: the partner's sign
+            const T Ap = signFlip(kap, (unsigned)__popc((unsigned)u & zv));  // for out_p
+            const T ur = x.re[u], ui = x.im[u], pr = x.re[p], pi = x.im[p];
+            if constexpr (KIND == 1) {
+                x.re[u] = c * ur + Bu * pr;
+                x.im[u] = c * ui + Bu * pi;
+                x.re[p] = c * pr + Ap * ur;
+                x.im[p] = c * pi + Ap * ui;
+            } else if constexpr (KIND == 2) {
+                x.re[u] = c * ur - Bu * pi;
+                x.im[u] = c * ui + Bu * pr;
+                x.re[p] = c * pr - Ap * ui;
+                x.im[p] = c * pi + Ap * ur;
+            } else if constexpr (KIND == 4) {
+                x.re[u] = ur + Bu * pr;
+                x.im[u] = ui + Bu * pi;
+                x.re[p] = pr + Ap * ur;
+                x.im[p] = pi + Ap * ui;
+            } else {
+                x.re[u] = ur - Bu * pi;
+                x.im[u] = ui + Bu * pr;
+                x.re[p] = pr - Ap * ui;
+                x.im[p] = pi + Ap * ur;
+            }
+        }
+    }
+}
+
+template <typename T, int E, int KIND>
+__device__ __forceinline__ void cosetExchange(CosetRegs<T, E> &x, unsigned m, T c, T kap, unsigned zv) {
+    switch (m) {
+    case 1: cosetPairs<T, E, 1, KIND>(x, c, kap, zv); break;
+    case 2: cosetPairs<T, E, 2, KIND>(x, c, kap, zv); break;
+    case 3: cosetPairs<T, E, 3, KIND>(x, c, kap, zv); break;
+    case 4: cosetPairs<T, E, 4, KIND>(x, c, kap, zv); break;
+    case 5: cosetPairs<T, E, 5, KIND>(x, c, kap, zv); break;
+    case 6: cosetPairs<T, E, 6, KIND>(x, c, kap, zv); break;
+    case 7: cosetPairs<T, E, 7, KIND>(x, c, kap, zv); break;
+    case 8: cosetPairs<T, E, 8, KIND>(x, c, kap, zv); break;
+    case 9: cosetPairs<T, E, 9, KIND>(x, c, kap, zv); break;
+    case 10: cosetPairs<T, E, 10, KIND>(x, c, kap, zv); break;
+    case 11: cosetPairs<T, E, 11, KIND>(x, c, kap, zv); break;
+    case 12: cosetPairs<T, E, 12, KIND>(x, c, kap, zv); break;
+    case 13: cosetPairs<T, E, 13, KIND>(x, c, kap, zv); break;
+    case 14: cosetPairs<T, E, 14, KIND>(x, c, kap, zv); break;
+    default: cosetPairs<T, E, 15, KIND>(x, c, kap, zv); break;
+    }
+}
+
+// x_u *= c - i kap s(u) (TAN: c = 1)
+template <typename T, int E, bool TAN>
+__device__ __forceinline__ void cosetDiagonal(CosetRegs<T, E> &x, T c, T kap, unsigned zv) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+        const T A = signFlip(kap, (unsigned)__popc((unsigned)u & zv));
+        const T vr = x.re[u], vi = x.im[u];
+        x.re[u] = TAN ? vr + A * vi : c * vr + A * vi;
+        x.im[u] = TAN ? vi - A * vr : c * vi - A * vr;
+    }
+}
+
+// Tile staging, prefetch and the final scaled store as in kPauliBlock; the
+// groups replace its passes.
+template <typename T, bool PREFETCH>
+__global__ void __launch_bounds__(kBlock) kPauliCoset(Amp<T> *__restrict__ a, unsigned numTiles, PauliTile tile,
+                                                      const CosetGroupDev *__restrict__ groups, int numGroups,
+                                                      const CosetExpDev *__restrict__ ops, double scale) {
+    constexpr int R = cosetBits<T>();
+    constexpr int E = 1 << R;   // amplitudes per thread; the tile is kBlock * E
+    extern __shared__ unsigned char smem[];
+    Amp<T> *st = reinterpret_cast<Amp<T> *>(smem);
+    const unsigned tx = threadIdx.x;
+    uint64_t dt = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if ((tx >> j) & 1) dt |= 1ull << tile.pos[j];
+    Amp<T> next[PREFETCH ? E : 1];
+    unsigned w = blockIdx.x;
+    if (PREFETCH) {
+        const Amp<T> *ns = a + (insertZeros((uint64_t)w, tile.fixed) | dt);
+#pragma unroll
+        for (int k = 0; k < E; ++k) next[k] = ns[tile.hi[k]];
+    }
+    for (; w < numTiles; w += PREFETCH ? gridDim.x : numTiles) {
+        const uint64_t base = insertZeros((uint64_t)w, tile.fixed);
+        Amp<T> *src = a + (base | dt);
+#pragma unroll
+        for (int k = 0; k < E; ++k) st[tx + (k << 8)] = PREFETCH ? next[k] : src[tile.hi[k]];
+        __syncthreads();
+        if (PREFETCH && w + gridDim.x < numTiles) {
+            const Amp<T> *ns = a + (insertZeros((uint64_t)(w + gridDim.x), tile.fixed) | dt);
+#pragma unroll
+            for (int k = 0; k < E; ++k) next[k] = ns[tile.hi[k]];
+        }
+        for (int g = 0; g < numGroups; ++g) {
+            const CosetGroupDev &grp = groups[g];
+            unsigned r = tx;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const int p = grp.piv[j];
+                r = ((r >> p) << (p + 1)) | (r & ((1u << p) - 1u));
+            }
+            CosetRegs<T, E> x;
+#pragma unroll
+            for (int v = 0; v < E; ++v) {
+                const Amp<T> t = st[r ^ grp.C[v]];
+                x.re[v] = t.x;
+                x.im[v] = t.y;
+            }
+            const int first = grp.first, count = grp.count;
+            for (int o = 0; o < count; ++o) {
+                const CosetExpDev e = ops[first + o];
+                const unsigned p0 = (unsigned)__popcll(base & e.z) + (unsigned)__popc(r & e.lz);
+                const T kap = signFlip((T)e.kap, p0);
+                const T c = (T)e.c;
+                switch (e.kind) {
+                case 0: cosetDiagonal<T, E, false>(x, c, kap, e.zv); break;
+                case 1: cosetExchange<T, E, 1>(x, e.m, c, kap, e.zv); break;
+                case 2: cosetExchange<T, E, 2>(x, e.m, c, kap, e.zv); break;
+                case 3: cosetDiagonal<T, E, true>(x, c, kap, e.zv); break;
+                case 4: cosetExchange<T, E, 4>(x, e.m, c, kap, e.zv); break;
+                default: cosetExchange<T, E, 5>(x, e.m, c, kap, e.zv); break;
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < E; ++v) st[r ^ grp.C[v]] = Amp<T>{x.re[v], x.im[v]};
+            __syncthreads();
+        }
+        const T sc = (T)scale;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const Amp<T> v = st[tx + (k << 8)];
+            src[tile.hi[k]] = Amp<T>{sc * v.x, sc * v.y};
+        }
+        if (PREFETCH) __syncthreads();  // the next tile overwrites the LDS copy
+    }
+}
+
 constexpr int kMaxBlockOps = 1 << 16;  // rotations per block (bounds the host's records only)
 
 struct Block {
     PauliTile tile;
-    int firstPass, numPasses;
+    int firstPass, numPasses;   // coset form: its groups
     int firstOp, numOps;
     int single;        // >= 0: one rotation too wide for a tile, applied alone (index into seq)
+    bool coset;        // coset form (full tiles) or pass form
     double scale;      // product of cos(theta/2) over the block's tangent-form rotations
 };
+
+// Split a block's rotations (tile-local X/Y masks lx, z masks lz) into coset
+// groups of at most R exchange dimensions; fills the ops' m and zv.
+void buildCosetGroups(const std::vector<unsigned> &lx, std::vector<CosetExpDev> &ops, int firstOp, int tb, int R,
+                      std::vector<CosetGroupDev> &groups) {
+    const int count = (int)lx.size();
+    int start = 0;
+    while (start < count) {
+        // grow the group while the masks' span stays within R dimensions
+        std::vector<unsigned> ech;  // echelon form by highest bit, for the membership test
+        int end = start;
+        for (; end < count; ++end) {
+            unsigned v = lx[end];
+            for (unsigned e : ech)
+                if ((v ^ e) < v) v ^= e;
+            if (v == 0) continue;
+            if ((int)ech.size() == R) break;
+            ech.push_back(v);
+            std::sort(ech.begin(), ech.end(), [](unsigned p, unsigned q) { return p > q; });
+        }
+        // reduced echelon basis (pivot = highest bit), extended by unit
+        // vectors at the highest free tile bits to exactly R vectors
+        std::vector<unsigned> b;
+        std::vector<int> piv;
+        auto add = [&](unsigned v) {
+            for (size_t j = 0; j < b.size(); ++j)
+                if ((v >> piv[j]) & 1) v ^= b[j];
+            const int p = 31 - __builtin_clz(v);
+            for (size_t j = 0; j < b.size(); ++j)
+                if ((b[j] >> p) & 1) b[j] ^= v;
+            b.push_back(v);
+            piv.push_back(p);
+        };
+        for (unsigned e : ech) add(e);
+        for (int p = tb - 1; p >= 0 && (int)b.size() < R; --p)
+            if (std::find(piv.begin(), piv.end(), p) == piv.end()) add(1u << p);
+        // order the basis by pivot, ascending
+        std::vector<int> ord(R);
+        for (int j = 0; j < R; ++j) ord[j] = j;
+        std::sort(ord.begin(), ord.end(), [&](int p, int q) { return piv[p] < piv[q]; });
+        CosetGroupDev g{};
+        g.first = start;
+        g.count = end - start;
+        unsigned bs[kMaxCosetBits];
+        for (int j = 0; j < R; ++j) {
+            g.piv[j] = piv[ord[j]];
+            bs[j] = b[ord[j]];
+        }
+        for (int v = 0; v < (1 << R); ++v) {
+            unsigned cv = 0;
+            for (int j = 0; j < R; ++j)
+                if ((v >> j) & 1) cv ^= bs[j];
+            g.C[v] = cv;
+        }
+        for (int k = start; k < end; ++k) {
+            CosetExpDev &d = ops[firstOp + k];
+            unsigned m = 0, back = 0;
+            for (int j = 0; j < R; ++j)
+                if ((lx[k] >> g.piv[j]) & 1) {
+                    m |= 1u << j;
+                    back ^= bs[j];
+                }
+            if (back != lx[k]) {
+                std::fprintf(stderr, "libquest_mi355x: Pauli coset basis does not span a rotation's mask\n");
+                std::abort();
+            }
+            d.m = m;
+            d.zv = 0;
+            for (int j = 0; j < R; ++j) d.zv |= ((unsigned)__builtin_popcount(bs[j] & d.lz) & 1u) << j;
+        }
+        groups.push_back(g);
+        start = end;
+    }
+}
 
 }  // namespace
 
@@ -238,8 +504,12 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps) {
     const int n = q->numQubits;
     const int tb = std::min(n, q->prec == FP64 ? 11 : 12);  // 32 KiB of LDS per tile
     const int maxX = std::max(1, tb - std::min(tb, 4));     // keep >= 4 low bits for contiguous runs
+    const int R = q->prec == FP64 ? cosetBits<double>() : cosetBits<float>();
+    const bool coset = tb == 8 + R;                           // full tiles: the coset form
     std::vector<PauliExpDev> ops;
     std::vector<PassDev> passes;
+    std::vector<CosetExpDev> cops;
+    std::vector<CosetGroupDev> groups;
     std::vector<Block> blocks;
     int i = 0;
     while (i < count) {
@@ -248,7 +518,7 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps) {
             continue;
         }
         if (__builtin_popcountll(seq[i].x) > maxX) {
-            blocks.push_back(Block{PauliTile{}, 0, 0, 0, 0, i, 1.0});
+            blocks.push_back(Block{PauliTile{}, 0, 0, 0, 0, i, false, 1.0});
             ++i;
             continue;
         }
@@ -277,8 +547,10 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps) {
                 if ((k >> (j - 8)) & 1) g |= 1ull << blk.tile.pos[j];
             blk.tile.hi[k] = g;
         }
-        blk.firstPass = (int)passes.size();
-        blk.firstOp = (int)ops.size();
+        blk.coset = coset;
+        blk.firstPass = coset ? (int)groups.size() : (int)passes.size();
+        blk.firstOp = coset ? (int)cops.size() : (int)ops.size();
+        std::vector<unsigned> lxs;  // coset form: the rotations' tile-local X/Y masks
         int open = -1;  // pass accepting rotations
         for (int k = i; k < j; ++k) {
             const PauliExp &e = seq[k];
@@ -309,6 +581,11 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps) {
                 d.kap /= d.c;
                 blk.scale *= d.c;
             }
+            if (coset) {
+                cops.push_back(CosetExpDev{d.z, d.lz, 0, 0, d.kind, d.c, d.kap});
+                lxs.push_back(lx);
+                continue;
+            }
             const int rel = (int)ops.size() - blk.firstOp;  // passes index the block's own ops
             if (open < 0) {
                 passes.push_back(PassDev{lx, rel, 0, 0, {}});
@@ -321,6 +598,14 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps) {
             }
             ops.push_back(d);
             ++passes[open].count;
+        }
+        if (coset) {
+            buildCosetGroups(lxs, cops, blk.firstOp, blk.tile.tb, R, groups);
+            blk.numPasses = (int)groups.size() - blk.firstPass;
+            blk.numOps = (int)cops.size() - blk.firstOp;
+            if (blk.numPasses > 0) blocks.push_back(blk);
+            i = j;
+            continue;
         }
         blk.numPasses = (int)passes.size() - blk.firstPass;
         blk.numOps = (int)ops.size() - blk.firstOp;
@@ -347,14 +632,20 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps) {
         i = j;
     }
     if (blocks.empty()) return;
-    PauliExpDev *dOps = nullptr;
-    PassDev *dPasses = nullptr;
-    if (!ops.empty()) {
-        dOps = static_cast<PauliExpDev *>(devAlloc(ops.size() * sizeof(PauliExpDev)));
-        hostToDevice(dOps, ops.data(), ops.size() * sizeof(PauliExpDev));
-        dPasses = static_cast<PassDev *>(devAlloc(passes.size() * sizeof(PassDev)));
-        hostToDevice(dPasses, passes.data(), passes.size() * sizeof(PassDev));
-    }
+    std::vector<void *> owned;
+    auto upload = [&](const void *h, size_t bytes) -> void * {
+        if (bytes == 0) return nullptr;
+        void *d = devAlloc(bytes);
+        hostToDevice(d, h, bytes);
+        owned.push_back(d);
+        return d;
+    };
+    const PauliExpDev *dOps = static_cast<const PauliExpDev *>(upload(ops.data(), ops.size() * sizeof(PauliExpDev)));
+    const PassDev *dPasses = static_cast<const PassDev *>(upload(passes.data(), passes.size() * sizeof(PassDev)));
+    const CosetExpDev *dCops =
+        static_cast<const CosetExpDev *>(upload(cops.data(), cops.size() * sizeof(CosetExpDev)));
+    const CosetGroupDev *dGroups =
+        static_cast<const CosetGroupDev *>(upload(groups.data(), groups.size() * sizeof(CosetGroupDev)));
     const unsigned prefetchGrid = 4u * (unsigned)numCUs();
     for (int r = 0; r < reps; ++r) {
         for (const Block &b : blocks) {
@@ -364,23 +655,33 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seq, int count, int reps) {
                 continue;
             }
             const unsigned tiles = (unsigned)(q->numAmps >> b.tile.tb);
+            const size_t lds = (q->prec == FP64 ? sizeof(Amp<double>) : sizeof(Amp<float>)) << b.tile.tb;
+            if (b.coset) {
+                if (q->prec == FP64)
+                    hipLaunchKernelGGL((kPauliCoset<double, false>), dim3(tiles), dim3(kBlock), lds, hstream(),
+                                       (Amp<double> *)q->amps, tiles, b.tile, dGroups + b.firstPass, b.numPasses,
+                                       dCops + b.firstOp, b.scale);
+                else
+                    hipLaunchKernelGGL((kPauliCoset<float, true>), dim3(std::min(tiles, prefetchGrid)), dim3(kBlock),
+                                       lds, hstream(), (Amp<float> *)q->amps, tiles, b.tile, dGroups + b.firstPass,
+                                       b.numPasses, dCops + b.firstOp, b.scale);
+                continue;
+            }
             // pairs per thread: 2^tb / 2 / kBlock at the full tile size (8 FP32, 4 FP64)
             if (q->prec == FP64)
-                hipLaunchKernelGGL((kPauliBlock<double, 4, false>), dim3(tiles), dim3(kBlock),
-                                   (size_t)sizeof(Amp<double>) << b.tile.tb, hstream(), (Amp<double> *)q->amps,
-                                   tiles, b.tile, dPasses + b.firstPass, b.numPasses, dOps + b.firstOp, b.scale);
+                hipLaunchKernelGGL((kPauliBlock<double, 4, false>), dim3(tiles), dim3(kBlock), lds, hstream(),
+                                   (Amp<double> *)q->amps, tiles, b.tile, dPasses + b.firstPass, b.numPasses,
+                                   dOps + b.firstOp, b.scale);
             else
                 hipLaunchKernelGGL((kPauliBlock<float, 8, true>), dim3(std::min(tiles, prefetchGrid)),
-                                   dim3(kBlock), (size_t)sizeof(Amp<float>) << b.tile.tb, hstream(),
-                                   (Amp<float> *)q->amps, tiles, b.tile, dPasses + b.firstPass, b.numPasses,
-                                   dOps + b.firstOp, b.scale);
+                                   dim3(kBlock), lds, hstream(), (Amp<float> *)q->amps, tiles, b.tile,
+                                   dPasses + b.firstPass, b.numPasses, dOps + b.firstOp, b.scale);
         }
         checkLaunch("applyPauliSequence");
     }
-    if (dOps) {
+    if (!owned.empty()) {
         deviceSync();
-        devFree(dOps);
-        devFree(dPasses);
+        for (void *d : owned) devFree(d);
     }
 }
 

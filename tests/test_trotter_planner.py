@@ -334,10 +334,11 @@ def _wide_instance(n, seed):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,density", [(14, False), (17, False), (6, True), (7, True)])
+@pytest.mark.parametrize("n,density", [(9, False), (14, False), (17, False), (6, True), (7, True)])
 def test_fused_trotter_sweeps_match_reference_backend(gpu, cpuref, n, density):
     """applyTrotterCircuit runs its rotations as fused LDS-tile sweeps; against
-    the C restatement of QuEST's CPU backend (one rotation at a time)."""
+    the C restatement of QuEST's CPU backend (one rotation at a time). FP64
+    tiles hold 11 qubits: 9 qubits run the pass form, the rest the coset form."""
     from tests.cases import random_state
     inst = _wide_instance(n, seed=100 + n)
     for order, reps in ((1, 2), (2, 1), (4, 1)):
@@ -373,3 +374,23 @@ def test_fused_trotter_fp32_against_fp64(gpu):
     assert 1 - fid < 1e-6
     for q in states:
         gpu.destroy(q)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [10, 12, 13])
+def test_fused_trotter_fp32_amplitudes(gpu, n):
+    """FP32 tiles hold 12 qubits: 10 qubits run the pass form, 12 and 13 the
+    coset form; every amplitude against the FP64 run of the same circuit."""
+    from quantumlab_amd.quest import QMX_PREC_FP32
+    from tests.cases import random_state
+    inst = _wide_instance(n, seed=200 + n)
+    hb = H.HamilBatch([inst])
+    for order, reps in ((1, 2), (2, 1), (4, 1)):
+        outs = []
+        for prec in (None, QMX_PREC_FP32):
+            q = gpu.qureg(n, precision=prec)
+            gpu.set_state(q, random_state(n, seed=n))
+            gpu.lib.applyTrotterCircuit(q, hb[0], 0.37, order, reps)
+            outs.append(gpu.state(q))
+            gpu.destroy(q)
+        assert np.max(np.abs(outs[0] - outs[1])) < 2e-6, (n, order)
