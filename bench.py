@@ -305,8 +305,9 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                          "sweeps": int(sum(x[6] for x in rows)), "bytes_per_sweep": 2.0 * (1 << n) * 8,
                          "note": "read + write of the FP32 register per fused block, over the learned solves' wall "
-                                 "time, host planning and launch gaps included; one LDS round trip per coset group "
-                                 "(about 1.7 per block) on top of the tile's HBM read + write"},
+                                 "time, host planning and launch gaps included; commuting rotations are packed into "
+                                 "fewer, fuller blocks, so a sweep carries more rotations (and LDS round trips, one "
+                                 "per coset group) per HBM read + write"},
             "choices": [[x[1], x[2]] for x in rows],
             "step_doubling_reps": [[x[5][2][1], x[5][4][1]] for x in rows]}, rows
 
