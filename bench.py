@@ -197,11 +197,12 @@ def run_density(qe, n, layers, seed):
     qe.destroy(pure)
     a, b = outs["fp64"], outs["fp32"]
     res.update(outs)
-    res["roofline"] = {"bound": "hbm", "kernel": "kDense<double, 4> (composed superoperator sweeps of the 2n-qubit vector)",
+    res["roofline"] = {"bound": "hbm", "kernel": "kDensePair<double, 4, 4> (two composed superoperators per sweep of the 2n-qubit vector)",
                        "achieved": a["gb_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(a["gb_per_s"] / HBM_PEAK_GBS, 4), "traffic": None,
                        "note": "read + write of every entry per sweep (2 * 4^n * 16 B) over the leg's wall time; "
-                               "all operations compose, so the sweeps are the only device traffic. "
+                               "all operations compose, so the sweeps are the only device traffic; a sweep applies "
+                               "two 2-qubit superoperators (32 complex FMAs per entry). "
                                "per_operation_gb_per_s counts each gate and channel as its own sweep, as the "
                                "reference executes them"}
     res["fp32_vs_fp64"] = {"purity_error": abs(a["purity"] - b["purity"]),

@@ -124,6 +124,94 @@ __global__ void __launch_bounds__(64) kDenseLds(Amp<T> *__restrict__ a, DenseLds
     }
 }
 
+// Two matrices on disjoint bit sets A (BA bits) and B (BB bits) in one sweep:
+// they commute, and a workgroup can hold every A x B combination. The tile is
+// 4096 entries — the A and B bits plus the lowest other bits — staged in LDS
+// with coalesced loads (tile index j = thread + 256 i, so a load instruction
+// covers the tile's lowest bits, a prefix of the global bits). Each thread then
+// applies A to its 16 / 2^BA A-groups (a group: the 2^BA entries that differ
+// only in A bits), and after a barrier B to its B-groups, reading and writing
+// only its own groups in each phase; the tile goes back with coalesced stores.
+// One HBM read + write for two operations (the density registers' paired
+// superoperator sweeps).
+constexpr int kPairTileBits = 12;
+
+struct PairPhase {
+    int thr[8];          // tile bit of thread-index bit j
+    unsigned g[4];       // tile-index offset of the thread's group g
+    unsigned e[16];      // tile-index offset of group entry e (bit i of e -> the matrix's bit i)
+};
+
+struct DensePairSpec {
+    BitList tile;        // the tile's global bit positions, ascending
+    uint64_t lowOff[8];  // global offset of tile-index bit j < 8
+    uint64_t hi[16];     // global offset of tile index 256 i
+    PairPhase pa, pb;
+};
+
+template <typename T, int BX>
+__device__ __forceinline__ void pairPhase(Amp<T> *lds, const PairPhase &ph, const Amp<T> *__restrict__ M) {
+    constexpr int PS = sizeof(T) == 8 ? 3 : 4;
+    constexpr int E = 1 << BX, G = 16 / E;
+    const unsigned t = threadIdx.x;
+    unsigned bt = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) bt |= ((t >> j) & 1u) << ph.thr[j];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const unsigned gb = bt | ph.g[g];
+        T xr[E], xi[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const unsigned j = gb | ph.e[e];
+            const Amp<T> v = lds[j + (j >> PS)];
+            xr[e] = v.x;
+            xi[e] = v.y;
+        }
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            T yr = 0, yi = 0;
+#pragma unroll
+            for (int c = 0; c < E; ++c) {
+                const Amp<T> m = M[r * E + c];
+                yr = fma(m.x, xr[c], fma(-m.y, xi[c], yr));
+                yi = fma(m.x, xi[c], fma(m.y, xr[c], yi));
+            }
+            const unsigned j = gb | ph.e[r];
+            lds[j + (j >> PS)] = Amp<T>{yr, yi};
+        }
+    }
+}
+
+template <typename T, int BA, int BB>
+__global__ void __launch_bounds__(kBlock) kDensePair(Amp<T> *__restrict__ a, DensePairSpec sp,
+                                                     const Amp<T> *__restrict__ MA, const Amp<T> *__restrict__ MB) {
+    constexpr int PS = sizeof(T) == 8 ? 3 : 4;
+    extern __shared__ unsigned char smemRaw[];
+    Amp<T> *lds = reinterpret_cast<Amp<T> *>(smemRaw);
+    const unsigned t = threadIdx.x;
+    uint64_t dt = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if ((t >> j) & 1) dt |= sp.lowOff[j];
+    Amp<T> *src = a + (insertZeros((uint64_t)xcdBlock(), sp.tile) | dt);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const unsigned j = t + 256u * i;
+        lds[j + (j >> PS)] = src[sp.hi[i]];
+    }
+    __syncthreads();
+    pairPhase<T, BA>(lds, sp.pa, MA);
+    __syncthreads();
+    pairPhase<T, BB>(lds, sp.pb, MB);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const unsigned j = t + 256u * i;
+        src[sp.hi[i]] = lds[j + (j >> PS)];
+    }
+}
+
 struct MatrixRing {
     char *dev = nullptr, *host = nullptr;
     hipEvent_t ev[kRing];
@@ -219,7 +307,102 @@ void denseLds(QuregDev *q, const int *bits, int B, const int *sortedBits, const 
     checkLaunch("applyDense (LDS)");
 }
 
+// tile-index layout of one phase: the matrix's bits sit at tile positions
+// pos[0..BX-1]; a thread's 8 index bits and its groups take the other bits
+PairPhase pairLayout(const int *pos, int BX) {
+    PairPhase ph{};
+    int other[kPairTileBits], no = 0;
+    for (int b = 0; b < kPairTileBits; ++b)
+        if (std::find(pos, pos + BX, b) == pos + BX) other[no++] = b;
+    for (int j = 0; j < 8; ++j) ph.thr[j] = other[j];
+    for (int g = 0; g < (16 >> BX); ++g) {
+        unsigned o = 0;
+        for (int j = 0; 8 + j < no; ++j)
+            if ((g >> j) & 1) o |= 1u << other[8 + j];
+        ph.g[g] = o;
+    }
+    for (int e = 0; e < (1 << BX); ++e) {
+        unsigned o = 0;
+        for (int i = 0; i < BX; ++i)
+            if ((e >> i) & 1) o |= 1u << pos[i];
+        ph.e[e] = o;
+    }
+    return ph;
+}
+
+template <typename T, int BA, int BB>
+void launchDensePair(QuregDev *q, const DensePairSpec &sp, const Cplx *MA, const Cplx *MB, hipStream_t st) {
+    constexpr int PS = sizeof(T) == 8 ? 3 : 4;
+    constexpr int entries = 1 << kPairTileBits;
+    const size_t lds = (size_t)(entries + (entries >> PS)) * sizeof(Amp<T>);
+    static bool ldsAllowed = false;
+    if (!ldsAllowed) {
+        if (!allowDynamicLds(reinterpret_cast<const void *>(&kDensePair<T, BA, BB>), lds, "applyDensePair"))
+            std::abort();
+        ldsAllowed = true;
+    }
+    // both matrices in one staging slot: A at 0, B at 256
+    Cplx both[2 * 256];
+    std::copy(MA, MA + (1 << (2 * BA)), both);
+    std::copy(MB, MB + (1 << (2 * BB)), both + 256);
+    const Amp<T> *m = stageMatrix<T>(both, 512, st);
+    const unsigned grid = (unsigned)(q->numAmps >> kPairTileBits);
+    hipLaunchKernelGGL((kDensePair<T, BA, BB>), dim3(grid), dim3(kBlock), lds, st, (Amp<T> *)q->amps, sp, m,
+                       m + 256);
+}
+
 }  // namespace
+
+bool densePairSupported(const QuregDev *q, int BA, int BB) {
+    return (BA == 2 || BA == 4) && (BB == 2 || BB == 4) && q->numQubits >= kPairTileBits;
+}
+
+void applyDensePair(QuregDev *q, const int *bitsA, int BA, const Cplx *MA, const int *bitsB, int BB,
+                    const Cplx *MB) {
+    if (!densePairSupported(q, BA, BB)) {
+        std::fprintf(stderr, "libquest_mi355x: applyDensePair: unsupported shape (%d, %d bits on %d qubits)\n", BA,
+                     BB, q->numQubits);
+        std::abort();
+    }
+    if (BA < BB) {  // the matrices commute: the larger one first
+        std::swap(bitsA, bitsB);
+        std::swap(BA, BB);
+        std::swap(MA, MB);
+    }
+    // tile bits: A, B and the lowest other bits
+    uint64_t tm = 0;
+    for (int i = 0; i < BA; ++i) tm |= 1ull << bitsA[i];
+    for (int i = 0; i < BB; ++i) tm |= 1ull << bitsB[i];
+    for (int b = 0; b < q->numQubits && __builtin_popcountll(tm) < kPairTileBits; ++b) tm |= 1ull << b;
+    DensePairSpec sp{};
+    int tpos[kPairTileBits];
+    for (int b = 0, j = 0; b < q->numQubits; ++b)
+        if ((tm >> b) & 1) {
+            tpos[j] = b;
+            sp.tile.pos[j++] = b;
+        }
+    sp.tile.n = kPairTileBits;
+    for (int j = 0; j < 8; ++j) sp.lowOff[j] = 1ull << tpos[j];
+    for (int i = 0; i < 16; ++i) {
+        uint64_t o = 0;
+        for (int k = 0; k < 4; ++k)
+            if ((i >> k) & 1) o |= 1ull << tpos[8 + k];
+        sp.hi[i] = o;
+    }
+    auto local = [&](int bit) { return (int)(std::find(tpos, tpos + kPairTileBits, bit) - tpos); };
+    int pa[4], pb[4];
+    for (int i = 0; i < BA; ++i) pa[i] = local(bitsA[i]);
+    for (int i = 0; i < BB; ++i) pb[i] = local(bitsB[i]);
+    sp.pa = pairLayout(pa, BA);
+    sp.pb = pairLayout(pb, BB);
+    hipStream_t st = hstream();
+    QMX_DISPATCH(q->prec, T, {
+        if (BA == 4 && BB == 4) launchDensePair<T, 4, 4>(q, sp, MA, MB, st);
+        else if (BA == 4) launchDensePair<T, 4, 2>(q, sp, MA, MB, st);
+        else launchDensePair<T, 2, 2>(q, sp, MA, MB, st);
+    });
+    checkLaunch("applyDensePair");
+}
 
 bool denseBitsSupported(int B) { return B >= 2 && B <= kMaxB; }
 

@@ -143,6 +143,28 @@ void sweepBlock(QuregDev *q, const Block &b) {
     launchSuperOp(q, b.t, b.k, b.S);
 }
 
+// blocks act on disjoint qubits and commute: sweep them two at a time, both
+// superoperators in one pass (applyDensePair), when the register allows
+void sweepBlocks(QuregDev *q, const std::vector<const Block *> &bs) {
+    size_t i = 0;
+    for (; i + 1 < bs.size(); i += 2) {
+        const Block &a = *bs[i], &b = *bs[i + 1];
+        if (!densePairSupported(q, 2 * a.k, 2 * b.k)) break;
+        int ba[2 * kPendMax], bb[2 * kPendMax];
+        for (int j = 0; j < a.k; ++j) {
+            ba[j] = a.t[j];
+            ba[a.k + j] = a.t[j] + q->numQubitsRep;
+        }
+        for (int j = 0; j < b.k; ++j) {
+            bb[j] = b.t[j];
+            bb[b.k + j] = b.t[j] + q->numQubitsRep;
+        }
+        ++g_passes;
+        applyDensePair(q, ba, 2 * a.k, a.S, bb, 2 * b.k, b.S);
+    }
+    for (; i < bs.size(); ++i) sweepBlock(q, *bs[i]);
+}
+
 void flushPending() {
     setPendingWork(nullptr);
     if (!g_pend.q) return;
@@ -150,7 +172,9 @@ void flushPending() {
     std::vector<Block> blocks;
     blocks.swap(g_pend.blocks);
     g_pend.q = nullptr;
-    for (const Block &b : blocks) sweepBlock(q, b);
+    std::vector<const Block *> bs;
+    for (const Block &b : blocks) bs.push_back(&b);
+    sweepBlocks(q, bs);
 }
 
 // sub-operator s on subT (ks qubits, all in T) written into the 4^K x 4^K
@@ -231,12 +255,29 @@ void densSuperOp(QuregDev *q, const int *targets, int k, const Cplx *S) {
             if (bl[i].k == 1) into = i;
     }
     if (into < 0 && hits > 0) {
-        // the blocks this operation would bridge are swept first, in order
+        // the blocks this operation would bridge are swept first; an odd one
+        // out takes the oldest other pending block along in the same pass (in
+        // a brick layer that block is the next gate's to sweep)
+        std::vector<char> sweep(bl.size(), 0);
+        int bridged = 0;
+        for (size_t i = 0; i < bl.size(); ++i)
+            if (touches(bl[i])) {
+                sweep[i] = 1;
+                ++bridged;
+            }
+        if (bridged % 2 == 1 && densePairSupported(q, 4, 4))
+            for (size_t i = 0; i < bl.size(); ++i)
+                if (!sweep[i]) {
+                    sweep[i] = 1;
+                    break;
+                }
+        std::vector<const Block *> swept;
+        for (size_t i = 0; i < bl.size(); ++i)
+            if (sweep[i]) swept.push_back(&bl[i]);
+        sweepBlocks(q, swept);
         std::vector<Block> keep;
-        for (Block &b : bl) {
-            if (touches(b)) sweepBlock(q, b);
-            else keep.push_back(b);
-        }
+        for (size_t i = 0; i < bl.size(); ++i)
+            if (!sweep[i]) keep.push_back(bl[i]);
         bl.swap(keep);
     }
     if (into < 0) {
