@@ -127,6 +127,49 @@ def run_fp32_fused(qe, n, plans, warmup, steps, k):
     return q, wall, blocks, norm2
 
 
+def run_fp32_large(qe, n, seed, k):
+    """One period of the brick circuit on an n-qubit FP32 register (33 qubits:
+    64 GiB) through fused blocks, timed; then the same gates one by one on an
+    n-qubit FP64 register (128 GiB at 33 qubits) as its reference."""
+    from quantumlab_amd import workloads as W
+    from quantumlab_amd.quest import QMX_PREC_FP32
+    gates = period_gates(n, 0, seed)
+    q32 = qe.qureg(n, precision=QMX_PREC_FP32)
+    c = W.build_circuit(qe, n, gates)
+    qe.qmxCircuitFuse(c, k)
+    blocks = qe.qmxCircuitNumBlocks(c)
+    qe.qmxSync()
+    t0 = time.perf_counter()
+    qe.qmxCircuitApply(q32, c, 1)
+    qe.qmxSync()
+    wall = time.perf_counter() - t0
+    qe.qmxCircuitDestroy(c)
+    q64 = qe.qureg(n)
+    qe.qmxSync()
+    t0 = time.perf_counter()
+    W.apply_gates(qe, q64, gates)
+    qe.qmxSync()
+    wall64 = time.perf_counter() - t0
+    ip = qe.calcInnerProduct(q64, q32)
+    fid = (ip.real ** 2 + ip.imag ** 2) / (qe.calcTotalProb(q64) * qe.calcTotalProb(q32))
+    maxabs = qe.qmxMaxAbsDiff(q64, q32)
+    qe.destroy(q32)
+    qe.destroy(q64)
+    N = 1 << n
+    gbs = blocks * 2.0 * N * 8 / wall / 1e9
+    log(f"fused fp32 at {n} qubits: {wall * 1e3:.1f} ms for {len(gates)} gates in {blocks} blocks ({gbs:.0f} GB/s); "
+        f"fp64 per-gate {wall64:.2f} s; 1-F={1 - fid:.3e}, max|da|={maxabs:.3e}")
+    return {"qubits": n, "state_gib": N * 8 / 2**30, "gates": len(gates), "blocks": blocks,
+            "ms_per_step": round(wall * 1e3, 3), "gates_per_s": round(len(gates) / wall, 2),
+            "fp64_per_gate_seconds": round(wall64, 3),
+            "fp64_per_gate_gb_per_s": round(W.gate_bytes(gates, n, 16) / wall64 / 1e9, 1),
+            "speedup_vs_fp64_per_gate": round(wall64 / wall, 3),
+            "fidelity_error": float(1 - fid), "max_abs_amp_error": float(maxabs),
+            "roofline": {"bound": "hbm", "kernel": "kFused5", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                         "mfma_tflops": round(blocks * N * 256 / wall / 1e12, 2)}}
+
+
 def cpu_baseline(n_cpu, seed):
     """QuEST's own CPU backend (oracle/_ref) on one 5-layer period at n_cpu qubits."""
     from quantumlab_amd import workloads as W
@@ -534,6 +577,8 @@ def main():
     ap.add_argument("--qubits", type=int, default=32)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--fuse-qubits", type=int, default=5)
+    ap.add_argument("--fp32-qubits", type=int, default=33,
+                    help="register size of the large FP32 fused run (0: skip)")
     ap.add_argument("--cpu-qubits", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
@@ -600,6 +645,8 @@ def main():
         qe.destroy(q32)
     if q64 is not None:
         qe.destroy(q64)
+    if not args.no_fused and args.steps > 0 and args.fp32_qubits > 0:
+        result["fused_fp32_large"] = run_fp32_large(qe, args.fp32_qubits, args.seed, args.fuse_qubits)
 
     if not args.no_density:
         result["density"] = run_density(qe, args.density_qubits, args.density_layers, args.seed)

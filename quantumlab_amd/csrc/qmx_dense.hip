@@ -34,7 +34,7 @@ template <typename T, int B, bool NT>
 __global__ void __launch_bounds__(kBlock) kDense(Amp<T> *__restrict__ a, long long groups, DenseSpec sp,
                                                  const Amp<T> *__restrict__ M) {
     constexpr int E = 1 << B;
-    const long long g = xcdBlock() * kBlock + threadIdx.x;
+    const long long g = flatBlock() * kBlock + threadIdx.x;
     if (g >= groups) return;
     uint64_t base = (uint64_t)g;
 #pragma unroll
@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(64) kDenseLds(Amp<T> *__restrict__ a, DenseLds
     extern __shared__ unsigned char smemRaw[];
     Amp<T> *lds = reinterpret_cast<Amp<T> *>(smemRaw);
     const int l = threadIdx.x;
-    uint64_t base = (uint64_t)xcdBlock() << 6;
+    uint64_t base = (uint64_t)flatBlock() << 6;
 #pragma unroll
     for (int j = 0; j < B; ++j) base = insertZero(base, sp.fix[j]);
     Amp<T> *src = a + base + l;
@@ -244,7 +244,7 @@ const Amp<T> *stageMatrix(const Cplx *M, int entries, hipStream_t st) {
 template <typename T, int B>
 void launchDense(QuregDev *q, long long groups, const DenseSpec &sp, bool nt, const Cplx *M, hipStream_t st) {
     const Amp<T> *m = stageMatrix<T>(M, 1 << (2 * B), st);
-    const unsigned grid = gridFor(groups, 1);
+    const dim3 grid = gridFor(groups, 1);
     if (nt)
         hipLaunchKernelGGL((kDense<T, B, true>), dim3(grid), dim3(kBlock), 0, st, (Amp<T> *)q->amps, groups, sp, m);
     else
@@ -257,7 +257,7 @@ void launchDenseLds(QuregDev *q, long long groups, const DenseLdsSpec &sp, const
     constexpr int E = 1 << B;
     constexpr int PS = sizeof(T) == 8 ? 3 : 4;
     const size_t lds = (size_t)(64 * E + ((64 * E) >> PS)) * sizeof(Amp<T>);
-    hipLaunchKernelGGL((kDenseLds<T, B>), dim3((unsigned)(groups >> 6)), dim3(64), lds, st, (Amp<T> *)q->amps, sp, m);
+    hipLaunchKernelGGL((kDenseLds<T, B>), gridOf(groups >> 6), dim3(64), lds, st, (Amp<T> *)q->amps, sp, m);
 }
 
 // the LDS form for targets among the bits whose runs are shorter than 128 B

@@ -27,7 +27,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) kCollapse(Amp<T> *__restrict__ a, long long work, int qubit,
                                                     int outcome, double renorm) {
     const uint64_t bit = 1ull << qubit;
-    long long w = (long long)blockIdx.x * kBlock * kPer + threadIdx.x;
+    long long w = flatBlockLinear() * kBlock * kPer + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         long long wj = w + (long long)j * kBlock;
@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(kBlock) kCollapse(Amp<T> *__restrict__ a, long
 template <typename T>
 __global__ void __launch_bounds__(kBlock) kDensCollapse(Amp<T> *__restrict__ a, long long n, int rowBit,
                                                         int colBit, int outcome, double prob) {
-    long long i = (long long)blockIdx.x * kBlock * kPer + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * kPer + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < kPer; ++j, i += kBlock) {
         if (i < n) {
@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(kBlock) kDensCollapse(Amp<T> *__restrict__ a, 
 template <typename T>
 __global__ void __launch_bounds__(kBlock) kDephase2(Amp<T> *__restrict__ a, long long n, int q1, int q2,
                                                     int shift, double retain) {
-    long long i = (long long)blockIdx.x * kBlock * kPer + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * kPer + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < kPer; ++j, i += kBlock) {
         if (i < n) {

@@ -74,6 +74,10 @@ __device__ __forceinline__ long long xcdBlock() {
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
+// workgroup index in a grid from gridFor / gridOf
+__device__ __forceinline__ long long flatBlock() { return (long long)blockIdx.y * gridDim.x + xcdBlock(); }
+__device__ __forceinline__ long long flatBlockLinear() { return (long long)blockIdx.y * gridDim.x + blockIdx.x; }
+
 // nontemporal accesses pay off once a wave's contiguous runs reach 128 B
 inline bool useNT(int lowestFixedBit, int prec) {
     return ((1ll << lowestFixedBit) * (prec == FP64 ? 16 : 8)) >= 128;
@@ -142,10 +146,22 @@ __device__ __forceinline__ void blockReduce(Acc (&acc)[NV], double (&out)[NV]) {
     }
 }
 
-inline unsigned gridFor(long long work, int perThread) {
-    long long per = (long long)kBlock * perThread;
-    long long g = (work + per - 1) / per;
-    return (unsigned)(g < 1 ? 1 : g);
+// A 1-D index space as a launch grid. A dispatch counts work-items in 32 bits
+// per dimension, so a 2^33-amplitude register cannot run one item per thread
+// on a 1-D grid: rows of at most 2^23 workgroups (2^31 work-items of kBlock),
+// as many rows as needed. Kernels take their workgroup index from
+// flatBlock() (XCD-aware within a row; the row width is a multiple of 8 when
+// there is more than one row) or flatBlockLinear().
+inline dim3 gridOf(long long blocks) {
+    constexpr long long kRow = 1ll << 23;
+    if (blocks < 1) blocks = 1;
+    if (blocks <= kRow) return dim3((unsigned)blocks);
+    return dim3((unsigned)kRow, (unsigned)((blocks + kRow - 1) / kRow));
+}
+
+inline dim3 gridFor(long long work, int perThread) {
+    const long long per = (long long)kBlock * perThread;
+    return gridOf((work + per - 1) / per);
 }
 
 }  // namespace qmx

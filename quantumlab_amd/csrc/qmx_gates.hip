@@ -40,7 +40,7 @@ template <typename T> struct M2 { C2<T> m00, m01, m10, m11; };
 template <typename T, bool CTRL, bool NT>
 __global__ void __launch_bounds__(kBlock) kMatrix2(Amp<T> *__restrict__ a, long long work, int target,
                                                    BitList fixed, uint64_t setMask, M2<T> u) {
-    const long long w = xcdBlock() * kBlock + threadIdx.x;
+    const long long w = flatBlock() * kBlock + threadIdx.x;
     if (w >= work) return;
     const uint64_t i0 = CTRL ? (insertZeros((uint64_t)w, fixed) | setMask) : insertZero((uint64_t)w, target);
     const uint64_t i1 = i0 | (1ull << target);
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(kBlock) kMatrix2(Amp<T> *__restrict__ a, long 
 template <typename T, bool NT>
 __global__ void __launch_bounds__(kBlock) kNot(Amp<T> *__restrict__ a, long long work, BitList fixed,
                                                uint64_t setMask, uint64_t targMask) {
-    const long long w = xcdBlock() * kBlock + threadIdx.x;
+    const long long w = flatBlock() * kBlock + threadIdx.x;
     if (w >= work) return;
     const uint64_t i = insertZeros((uint64_t)w, fixed) | setMask, k = i ^ targMask;
     const Amp<T> x = ldA<T, NT>(a + i), y = ldA<T, NT>(a + k);
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(kBlock) kNot(Amp<T> *__restrict__ a, long long
 template <typename T, bool NT>
 __global__ void __launch_bounds__(kBlock) kSwap(Amp<T> *__restrict__ a, long long work, BitList fixed,
                                                 uint64_t m1, uint64_t m2) {
-    const long long w = xcdBlock() * kBlock + threadIdx.x;
+    const long long w = flatBlock() * kBlock + threadIdx.x;
     if (w >= work) return;
     const uint64_t base = insertZeros((uint64_t)w, fixed);
     const uint64_t i = base | m2, k = base | m1;
@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(kBlock) kSwap(Amp<T> *__restrict__ a, long lon
 template <typename T, bool NT>
 __global__ void __launch_bounds__(kBlock) kPhaseAllOnes(Amp<T> *__restrict__ a, long long work,
                                                         BitList fixed, uint64_t mask, C2<T> t) {
-    const long long w = xcdBlock() * kBlock + threadIdx.x;
+    const long long w = flatBlock() * kBlock + threadIdx.x;
     if (w >= work) return;
     const uint64_t i = insertZeros((uint64_t)w, fixed) | mask;
     const Amp<T> v = ldA<T, NT>(a + i);
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(kBlock) kPhaseAllOnes(Amp<T> *__restrict__ a, 
 template <typename T, bool NT>
 __global__ void __launch_bounds__(kBlock) kRotZ(Amp<T> *__restrict__ a, long long work, BitList fixed,
                                                 uint64_t setMask, uint64_t targMask, T c, T s) {
-    const long long w = xcdBlock() * kBlock + threadIdx.x;
+    const long long w = flatBlock() * kBlock + threadIdx.x;
     if (w >= work) return;
     const uint64_t i = insertZeros((uint64_t)w, fixed) | setMask;
     const T f = parity64(i & targMask) ? -s : s;
@@ -110,7 +110,7 @@ template <typename T, bool NT>
 __global__ void __launch_bounds__(kBlock) kPauliRot(Amp<T> *__restrict__ a, long long work,
                                                     BitList fixed, uint64_t setMask, uint64_t xMask,
                                                     uint64_t zMask, C2<T> ph, T c, T s) {
-    const long long w = xcdBlock() * kBlock + threadIdx.x;
+    const long long w = flatBlock() * kBlock + threadIdx.x;
     if (w >= work) return;
     // coefficient on the partner: -i·s·ph (times ±1 from the Z parity)
     const T kre = s * ph.im, kim = -s * ph.re;

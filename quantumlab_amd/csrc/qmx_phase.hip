@@ -21,7 +21,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) kDiagApply(Amp<T> *__restrict__ a, long long n, unsigned long long rowMask,
                                                      const double *__restrict__ opRe,
                                                      const double *__restrict__ opIm) {
-    long long i = (long long)blockIdx.x * kBlock * kPer + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * kPer + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < kPer; ++j, i += kBlock) {
         if (i < n) {
@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(kBlock) kDiagApply(Amp<T> *__restrict__ a, lon
 __global__ void __launch_bounds__(kBlock) kDiagFromZ(double *__restrict__ re, double *__restrict__ im, long long n,
                                                      const unsigned long long *__restrict__ z,
                                                      const double *__restrict__ c, int numTerms) {
-    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const long long i = flatBlockLinear() * kBlock + threadIdx.x;
     if (i >= n) return;
     double e = 0;
     for (int t = 0; t < numTerms; ++t) e += (__popcll((unsigned long long)i & z[t]) & 1) ? -c[t] : c[t];
@@ -94,7 +94,7 @@ __device__ double namedPhase(const PhaseDev &p, const long long *x) {
 // amp *= exp(i phase) (QuEST_cpu.c:4251-4620)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) kPhaseFunc(Amp<T> *__restrict__ a, long long n, PhaseDev p) {
-    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const long long i = flatBlockLinear() * kBlock + threadIdx.x;
     if (i >= n) return;
     long long x[kMaxRegs];
     int flat = 0;

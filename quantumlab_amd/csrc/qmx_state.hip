@@ -11,7 +11,7 @@ namespace {
 
 template <typename T>
 __global__ void kFill(Amp<T> *a, long long n, T re, T im) {
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) a[i] = Amp<T>{re, im};
@@ -25,7 +25,7 @@ __global__ void kSetOne(Amp<T> *a, long long idx, T re, T im) {
 // QuEST_cpu.c:1655-1660: amp k = (2k + i(2k+1)) / 10
 template <typename T>
 __global__ void kDebug(Amp<T> *a, long long n) {
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) a[i] = Amp<T>{(T)((i * 2.0) / 10.0), (T)((i * 2.0 + 1.0) / 10.0)};
@@ -34,7 +34,7 @@ __global__ void kDebug(Amp<T> *a, long long n) {
 // QuEST_cpu.c:1617-1628: qubit fixed to `outcome`, others in |+>
 template <typename T>
 __global__ void kSingleQubit(Amp<T> *a, long long n, int qubit, int outcome, double norm) {
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) {
@@ -48,7 +48,7 @@ template <typename TR, typename TP>
 __global__ void kPureDensity(Amp<TR> *rho, const Amp<TP> *psi, int nRep) {
     long long dim = 1ll << nRep;
     long long n = dim * dim;
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) {
@@ -61,7 +61,7 @@ __global__ void kPureDensity(Amp<TR> *rho, const Amp<TP> *psi, int nRep) {
 
 template <typename TD, typename TS>
 __global__ void kCopy(Amp<TD> *d, const Amp<TS> *s, long long n) {
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) d[i] = Amp<TD>{(TD)s[i].x, (TD)s[i].y};
@@ -69,7 +69,7 @@ __global__ void kCopy(Amp<TD> *d, const Amp<TS> *s, long long n) {
 
 template <typename T>
 __global__ void kInterleave(Amp<T> *a, const double *re, const double *im, long long n) {
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) a[i] = Amp<T>{(T)re[i], (T)im[i]};
@@ -77,7 +77,7 @@ __global__ void kInterleave(Amp<T> *a, const double *re, const double *im, long 
 
 template <typename T>
 __global__ void kDeinterleave(const Amp<T> *a, double *re, double *im, long long n) {
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) {
@@ -91,7 +91,7 @@ __global__ void kDeinterleave(const Amp<T> *a, double *re, double *im, long long
 template <typename T>
 __global__ void kWeighted(Cplx f1, const Amp<T> *a, Cplx f2, const Amp<T> *b, Cplx fo, Amp<T> *o,
                           long long n) {
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) {
@@ -105,7 +105,7 @@ __global__ void kWeighted(Cplx f1, const Amp<T> *a, Cplx f2, const Amp<T> *b, Cp
 // combine = (1-p) combine + p other (QuEST_cpu.c:908-936)
 template <typename T>
 __global__ void kMixDens(Amp<T> *c, const Amp<T> *o, double p, long long n) {
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) {
@@ -118,7 +118,7 @@ __global__ void kMixDens(Amp<T> *c, const Amp<T> *o, double p, long long n) {
 
 template <typename T>
 __global__ void kScale(Amp<T> *a, long long n, double f) {
-    long long i = (long long)blockIdx.x * kBlock * 4 + threadIdx.x;
+    long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) a[i] = Amp<T>{(T)(a[i].x * f), (T)(a[i].y * f)};
