@@ -66,9 +66,9 @@ def main():
     # the state-vector legs only (per-gate FP64 and fused FP32), so the sweep
     # kernels' averages are those of the 32-qubit registers
     trace_cmd = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_trace -o bench -- "
-                 "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-swap --no-density --no-trotter")
+                 "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-swap --no-density --no-trotter --fp32-qubits 0")
     pmc_cmd = ("rocprofv3 --pmc {c} --output-format csv -d gpurun_out/prof_{d} -o bench -- "
-               "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-fused")
+               "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-fused --no-swap --no-density --no-trotter")
     fetch, calls = counter_means(args.fetch)
     write, _ = counter_means(args.write)
     n_amps = 1 << args.qubits
