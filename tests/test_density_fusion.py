@@ -335,3 +335,42 @@ def test_noisy_brick_at_twelve_qubits(gpu, cpuref, prec):
     want = cpuref.state(r)
     cpuref.destroy(r)
     assert np.max(np.abs(got - want)) < (TOL if prec == "fp64" else 2e-6)
+
+
+@pytest.mark.parametrize("shape", ["2q+2q", "2q+1q", "1q+2q", "low+high"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_paired_sweep_shapes(gpu, cpuref, shape, prec):
+    """Two pending blocks swept in one pass: superoperators on 4 + 4 and 4 + 2
+    bits (two one-qubit blocks never coexist: a lone one-qubit operation joins
+    the pending one-qubit block), with blocks on the lowest and highest
+    qubits. The read sweeps both in one pass; the state matches the oracle."""
+    from quantumlab_amd.quest import QMX_PREC_FP32, QMX_PREC_FP64
+    n = 7
+    rng = np.random.default_rng(hash(shape) % 1000)
+    blocks = {"2q+2q": ([1, 4], [2, 6]), "2q+1q": ([0, 5], [3]), "1q+2q": ([0], [4, 6]),
+              "low+high": ([0, 1], [5, 6])}[shape]
+    ops = []
+    for qs in blocks:
+        if len(qs) == 2:
+            ops += [("2qu", qs[0], qs[1], W.haar_unitary(rng, 4)), ("dep2", qs[0], qs[1], 0.3),
+                    ("damp", qs[1], 0.2)]
+        else:
+            ops += [("u", qs[0], W.haar_unitary(rng, 2)), ("depol", qs[0], 0.4), ("deph", qs[0], 0.1)]
+    q = gpu.qureg(n, density=True, precision=QMX_PREC_FP64 if prec == "fp64" else QMX_PREC_FP32)
+    r = cpuref.qureg(n, density=True)
+    for lib, reg in ((gpu, q), (cpuref, r)):
+        lib.initPlusState(reg)
+        lib.call("hadamard", reg, 3)
+        lib.calcPurity(reg)                                  # nothing pending before the blocks
+    s0 = fusion_stats(gpu)
+    for op in ops:
+        apply_op(gpu, q, op)
+        apply_op(cpuref, r, op)
+    got_p = gpu.calcPurity(q)
+    s1 = fusion_stats(gpu)
+    assert s1[0] - s0[0] == 1                                # both blocks in one sweep
+    got, want = gpu.state(q), cpuref.state(r)
+    assert abs(got_p - cpuref.calcPurity(r)) < (TOL if prec == "fp64" else 2e-6)
+    assert np.max(np.abs(got - want)) < (TOL if prec == "fp64" else 2e-6)
+    gpu.destroy(q)
+    cpuref.destroy(r)
