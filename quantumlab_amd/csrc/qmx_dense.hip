@@ -212,6 +212,165 @@ __global__ void __launch_bounds__(kBlock) kDensePair(Amp<T> *__restrict__ a, Den
     pairPhase<T, BB, false>(a, tileBase, lds, sp.pb, MB);
 }
 
+// FP32 pairs of 2-qubit superoperators (4 + 4 bits) on the matrix cores. The
+// VALU form spends 32 complex FMAs per entry, which at the non-packed FP32
+// rate takes longer than the sweep's HBM traffic. Here a 16x16 superoperator S
+// in real form R = [[Sr, -Si], [Si, Sr]] (32 x 32) multiplies 16 groups at a
+// time as 2 row tiles x 8 K-steps of v_mfma_f32_16x16x4f32:
+//   A (16 x 4): lane l holds R[16 rt + l%16][4 s + l/16], kept in registers;
+//   B (4 x 16): lane l holds entry 4 s' + l/16 of group l%16 (real part for
+//               K-step s', imaginary part for K-step s' + 4);
+//   D (16 x 16): lane l holds entries 4 (l/16) + v of group l%16 (real parts
+//               from row tile 0, imaginary from row tile 1).
+// The tile, its two phases and their memory paths are kDensePair's.
+typedef float pairF4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void pairFragments(const Amp<float> *__restrict__ M, int col, int q, float (&f)[2][8]) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const Amp<float> m = M[col * 16 + ((4 * s + q) & 15)];
+        f[0][s] = s < 4 ? m.x : -m.y;
+        f[1][s] = s < 4 ? m.y : m.x;
+    }
+}
+
+// element k of {v[0], v[1], v[2], v[3]} for a lane-dependent k in 0..3
+template <typename V>
+__device__ __forceinline__ V pick4(V v0, V v1, V v2, V v3, int k) {
+    return k == 0 ? v0 : k == 1 ? v1 : k == 2 ? v2 : v3;
+}
+
+// per-lane offsets of one phase: entry s of the lane's input (4 s + q) and
+// entry v of its output (4 q + v) of group col, in tile-index (LDS) or global
+// form; a batch b adds the uniform group-id bits 4-7 (batchBits)
+struct MfmaLayout {
+    unsigned t[4];     // tile index of the lane's LDS entries
+    uint64_t g[4];     // global offset of the lane's HBM entries
+};
+
+// LDS side from the entry set E0 (inputs: E0 = 4 s + q; outputs: 4 q + v),
+// HBM side from the other
+__device__ __forceinline__ MfmaLayout mfmaLayout(const PairPhase &ph, int col, int q, bool ldsIsOutput) {
+    unsigned lt = 0;
+    uint64_t lg = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if ((col >> j) & 1) {
+            lt |= 1u << ph.thr[j];
+            lg |= ph.thrG[j];
+        }
+    MfmaLayout L;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const unsigned inT = pick4(ph.e[4 * s], ph.e[4 * s + 1], ph.e[4 * s + 2], ph.e[4 * s + 3], q);
+        const unsigned outT = pick4(ph.e[s], ph.e[4 + s], ph.e[8 + s], ph.e[12 + s], q);
+        const uint64_t inG = pick4(ph.eG[4 * s], ph.eG[4 * s + 1], ph.eG[4 * s + 2], ph.eG[4 * s + 3], q);
+        const uint64_t outG = pick4(ph.eG[s], ph.eG[4 + s], ph.eG[8 + s], ph.eG[12 + s], q);
+        L.t[s] = lt | (ldsIsOutput ? outT : inT);
+        L.g[s] = lg | (ldsIsOutput ? inG : outG);
+    }
+    return L;
+}
+
+__device__ __forceinline__ void batchBits(const PairPhase &ph, int b, unsigned &bt, uint64_t &bg) {
+    bt = 0;
+    bg = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if ((b >> j) & 1) {
+            bt |= 1u << ph.thr[4 + j];
+            bg |= ph.thrG[4 + j];
+        }
+}
+
+// 16 groups: OUT = R X on the matrix cores; x holds the lane's 4 input entries
+__device__ __forceinline__ void mfmaGroups(const float (&f)[2][8], const Amp<float> (&x)[4], pairF4 &re,
+                                           pairF4 &im) {
+    re = pairF4{0.f, 0.f, 0.f, 0.f};
+    im = pairF4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        re = __builtin_amdgcn_mfma_f32_16x16x4f32(f[0][s], x[s].x, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x4f32(f[1][s], x[s].x, im, 0, 0, 0);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        re = __builtin_amdgcn_mfma_f32_16x16x4f32(f[0][s + 4], x[s].y, re, 0, 0, 0);
+        im = __builtin_amdgcn_mfma_f32_16x16x4f32(f[1][s + 4], x[s].y, im, 0, 0, 0);
+    }
+}
+
+constexpr int kMfmaBatches = 16 / (kBlock / 64);  // 16-group batches per wave and phase
+
+// a persistent grid: each workgroup walks tiles w = blockIdx.x + k gridDim.x.
+// The first superoperator runs on inputs already in registers; the next
+// tile's inputs are loaded as soon as those registers are free, so the loads
+// are in flight during the second superoperator and its HBM stores.
+__global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<float> *__restrict__ a, unsigned numTiles,
+                                                         DensePairSpec sp, const Amp<float> *__restrict__ MA,
+                                                         const Amp<float> *__restrict__ MB) {
+    constexpr int PS = 4;
+    extern __shared__ unsigned char smemRaw[];
+    Amp<float> *lds = reinterpret_cast<Amp<float> *>(smemRaw);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, q = lane >> 4;
+    float fa[2][8], fb[2][8];
+    pairFragments(MA, col, q, fa);
+    pairFragments(MB, col, q, fb);
+    const MfmaLayout LA = mfmaLayout(sp.pa, col, q, true), LB = mfmaLayout(sp.pb, col, q, false);
+    Amp<float> x[kMfmaBatches][4];
+    auto loadInputs = [&](unsigned tile) {
+        const Amp<float> *src = a + insertZeros((uint64_t)tile, sp.tile);
+#pragma unroll
+        for (int i = 0; i < kMfmaBatches; ++i) {
+            unsigned bt;
+            uint64_t bg;
+            batchBits(sp.pa, wave + i * (kBlock / 64), bt, bg);
+#pragma unroll
+            for (int s = 0; s < 4; ++s) x[i][s] = src[bg | LA.g[s]];
+        }
+    };
+    unsigned w = blockIdx.x;
+    if (w < numTiles) loadInputs(w);
+    for (; w < numTiles; w += gridDim.x) {
+        // first superoperator: registers -> LDS
+#pragma unroll
+        for (int i = 0; i < kMfmaBatches; ++i) {
+            unsigned bt;
+            uint64_t bg;
+            batchBits(sp.pa, wave + i * (kBlock / 64), bt, bg);
+            pairF4 re, im;
+            mfmaGroups(fa, x[i], re, im);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const unsigned j = bt | LA.t[v];
+                lds[j + (j >> PS)] = Amp<float>{re[v], im[v]};
+            }
+        }
+        if (w + gridDim.x < numTiles) loadInputs(w + gridDim.x);
+        __syncthreads();
+        // second superoperator: LDS -> HBM
+        Amp<float> *dst = a + insertZeros((uint64_t)w, sp.tile);
+#pragma unroll
+        for (int i = 0; i < kMfmaBatches; ++i) {
+            unsigned bt;
+            uint64_t bg;
+            batchBits(sp.pb, wave + i * (kBlock / 64), bt, bg);
+            Amp<float> y[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const unsigned j = bt | LB.t[s];
+                y[s] = lds[j + (j >> PS)];
+            }
+            pairF4 re, im;
+            mfmaGroups(fb, y, re, im);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) dst[bg | LB.g[v]] = Amp<float>{re[v], im[v]};
+        }
+        __syncthreads();  // the next tile's first phase overwrites the LDS copy
+    }
+}
+
 struct MatrixRing {
     char *dev = nullptr, *host = nullptr;
     hipEvent_t ev[kRing];
@@ -347,10 +506,13 @@ void launchDensePair(QuregDev *q, const DensePairSpec &sp, const Cplx *MA, const
     constexpr int PS = sizeof(T) == 8 ? 3 : 4;
     constexpr int entries = 1 << kPairTileBits;
     const size_t lds = (size_t)(entries + (entries >> PS)) * sizeof(Amp<T>);
+    // FP32 4 + 4: the matrix-core form
+    constexpr bool mfma = sizeof(T) == 4 && BA == 4 && BB == 4;
+    const void *kern = mfma ? reinterpret_cast<const void *>(&kDensePairMfma)
+                            : reinterpret_cast<const void *>(&kDensePair<T, BA, BB>);
     static bool ldsAllowed = false;
     if (!ldsAllowed) {
-        if (!allowDynamicLds(reinterpret_cast<const void *>(&kDensePair<T, BA, BB>), lds, "applyDensePair"))
-            std::abort();
+        if (!allowDynamicLds(kern, lds, "applyDensePair")) std::abort();
         ldsAllowed = true;
     }
     // both matrices in one staging slot: A at 0, B at 256
@@ -359,8 +521,12 @@ void launchDensePair(QuregDev *q, const DensePairSpec &sp, const Cplx *MA, const
     std::copy(MB, MB + (1 << (2 * BB)), both + 256);
     const Amp<T> *m = stageMatrix<T>(both, 512, st);
     const unsigned grid = (unsigned)(q->numAmps >> kPairTileBits);
-    hipLaunchKernelGGL((kDensePair<T, BA, BB>), dim3(grid), dim3(kBlock), lds, st, (Amp<T> *)q->amps, sp, m,
-                       m + 256);
+    if constexpr (mfma)  // persistent: 3 workgroups per CU (its occupancy)
+        hipLaunchKernelGGL(kDensePairMfma, dim3(std::min(grid, 3u * (unsigned)numCUs())), dim3(kBlock), lds, st,
+                           (Amp<float> *)q->amps, grid, sp, (const Amp<float> *)m, (const Amp<float> *)m + 256);
+    else
+        hipLaunchKernelGGL((kDensePair<T, BA, BB>), dim3(grid), dim3(kBlock), lds, st, (Amp<T> *)q->amps, sp, m,
+                           m + 256);
 }
 
 }  // namespace
