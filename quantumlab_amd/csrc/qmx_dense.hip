@@ -223,12 +223,26 @@ __global__ void __launch_bounds__(kBlock) kDensePair(Amp<T> *__restrict__ a, Den
 //   D (16 x 16): lane l holds entries 4 (l/16) + v of group l%16 (real parts
 //               from row tile 0, imaginary from row tile 1).
 // The tile, its two phases and their memory paths are kDensePair's.
-typedef float pairF4 __attribute__((ext_vector_type(4)));
+template <typename T> struct MfmaAcc { typedef T type __attribute__((ext_vector_type(4))); };
 
-__device__ __forceinline__ void pairFragments(const Amp<float> *__restrict__ M, int col, int q, float (&f)[2][8]) {
+__device__ __forceinline__ MfmaAcc<float>::type mfma16x16x4(float a, float b, MfmaAcc<float>::type c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ MfmaAcc<double>::type mfma16x16x4(double a, double b, MfmaAcc<double>::type c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// output row i of a lane's accumulator element v: FP32 4 (l/16) + v, FP64
+// l/16 + 4 v (measured on gfx950 against a host product)
+template <typename T> __device__ __forceinline__ int mfmaOutRow(int q, int v) {
+    return sizeof(T) == 4 ? 4 * q + v : q + 4 * v;
+}
+
+template <typename T>
+__device__ __forceinline__ void pairFragments(const Amp<T> *__restrict__ M, int col, int q, T (&f)[2][8]) {
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const Amp<float> m = M[col * 16 + ((4 * s + q) & 15)];
+        const Amp<T> m = M[col * 16 + ((4 * s + q) & 15)];
         f[0][s] = s < 4 ? m.x : -m.y;
         f[1][s] = s < 4 ? m.y : m.x;
     }
@@ -248,8 +262,9 @@ struct MfmaLayout {
     uint64_t g[4];     // global offset of the lane's HBM entries
 };
 
-// LDS side from the entry set E0 (inputs: E0 = 4 s + q; outputs: 4 q + v),
+// LDS side from the entry set E0 (inputs: E0 = 4 s + q; outputs: mfmaOutRow),
 // HBM side from the other
+template <typename T>
 __device__ __forceinline__ MfmaLayout mfmaLayout(const PairPhase &ph, int col, int q, bool ldsIsOutput) {
     unsigned lt = 0;
     uint64_t lg = 0;
@@ -263,9 +278,9 @@ __device__ __forceinline__ MfmaLayout mfmaLayout(const PairPhase &ph, int col, i
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const unsigned inT = pick4(ph.e[4 * s], ph.e[4 * s + 1], ph.e[4 * s + 2], ph.e[4 * s + 3], q);
-        const unsigned outT = pick4(ph.e[s], ph.e[4 + s], ph.e[8 + s], ph.e[12 + s], q);
+        const unsigned outT = sizeof(T) == 4 ? pick4(ph.e[s], ph.e[4 + s], ph.e[8 + s], ph.e[12 + s], q) : inT;
         const uint64_t inG = pick4(ph.eG[4 * s], ph.eG[4 * s + 1], ph.eG[4 * s + 2], ph.eG[4 * s + 3], q);
-        const uint64_t outG = pick4(ph.eG[s], ph.eG[4 + s], ph.eG[8 + s], ph.eG[12 + s], q);
+        const uint64_t outG = sizeof(T) == 4 ? pick4(ph.eG[s], ph.eG[4 + s], ph.eG[8 + s], ph.eG[12 + s], q) : inG;
         L.t[s] = lt | (ldsIsOutput ? outT : inT);
         L.g[s] = lg | (ldsIsOutput ? inG : outG);
     }
@@ -284,19 +299,20 @@ __device__ __forceinline__ void batchBits(const PairPhase &ph, int b, unsigned &
 }
 
 // 16 groups: OUT = R X on the matrix cores; x holds the lane's 4 input entries
-__device__ __forceinline__ void mfmaGroups(const float (&f)[2][8], const Amp<float> (&x)[4], pairF4 &re,
-                                           pairF4 &im) {
-    re = pairF4{0.f, 0.f, 0.f, 0.f};
-    im = pairF4{0.f, 0.f, 0.f, 0.f};
+template <typename T>
+__device__ __forceinline__ void mfmaGroups(const T (&f)[2][8], const Amp<T> (&x)[4], typename MfmaAcc<T>::type &re,
+                                           typename MfmaAcc<T>::type &im) {
+    re = typename MfmaAcc<T>::type{0, 0, 0, 0};
+    im = typename MfmaAcc<T>::type{0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        re = __builtin_amdgcn_mfma_f32_16x16x4f32(f[0][s], x[s].x, re, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x4f32(f[1][s], x[s].x, im, 0, 0, 0);
+        re = mfma16x16x4(f[0][s], x[s].x, re);
+        im = mfma16x16x4(f[1][s], x[s].x, im);
     }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        re = __builtin_amdgcn_mfma_f32_16x16x4f32(f[0][s + 4], x[s].y, re, 0, 0, 0);
-        im = __builtin_amdgcn_mfma_f32_16x16x4f32(f[1][s + 4], x[s].y, im, 0, 0, 0);
+        re = mfma16x16x4(f[0][s + 4], x[s].y, re);
+        im = mfma16x16x4(f[1][s + 4], x[s].y, im);
     }
 }
 
@@ -306,21 +322,22 @@ constexpr int kMfmaBatches = 16 / (kBlock / 64);  // 16-group batches per wave a
 // The first superoperator runs on inputs already in registers; the next
 // tile's inputs are loaded as soon as those registers are free, so the loads
 // are in flight during the second superoperator and its HBM stores.
-__global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<float> *__restrict__ a, unsigned numTiles,
-                                                         DensePairSpec sp, const Amp<float> *__restrict__ MA,
-                                                         const Amp<float> *__restrict__ MB) {
-    constexpr int PS = 4;
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<T> *__restrict__ a, unsigned numTiles, DensePairSpec sp,
+                                                         const Amp<T> *__restrict__ MA,
+                                                         const Amp<T> *__restrict__ MB) {
+    constexpr int PS = sizeof(T) == 8 ? 3 : 4;
     extern __shared__ unsigned char smemRaw[];
-    Amp<float> *lds = reinterpret_cast<Amp<float> *>(smemRaw);
+    Amp<T> *lds = reinterpret_cast<Amp<T> *>(smemRaw);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, q = lane >> 4;
-    float fa[2][8], fb[2][8];
+    T fa[2][8], fb[2][8];
     pairFragments(MA, col, q, fa);
     pairFragments(MB, col, q, fb);
-    const MfmaLayout LA = mfmaLayout(sp.pa, col, q, true), LB = mfmaLayout(sp.pb, col, q, false);
-    Amp<float> x[kMfmaBatches][4];
+    const MfmaLayout LA = mfmaLayout<T>(sp.pa, col, q, true), LB = mfmaLayout<T>(sp.pb, col, q, false);
+    Amp<T> x[kMfmaBatches][4];
     auto loadInputs = [&](unsigned tile) {
-        const Amp<float> *src = a + insertZeros((uint64_t)tile, sp.tile);
+        const Amp<T> *src = a + insertZeros((uint64_t)tile, sp.tile);
 #pragma unroll
         for (int i = 0; i < kMfmaBatches; ++i) {
             unsigned bt;
@@ -339,33 +356,33 @@ __global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<float> *__restrict_
             unsigned bt;
             uint64_t bg;
             batchBits(sp.pa, wave + i * (kBlock / 64), bt, bg);
-            pairF4 re, im;
+            typename MfmaAcc<T>::type re, im;
             mfmaGroups(fa, x[i], re, im);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const unsigned j = bt | LA.t[v];
-                lds[j + (j >> PS)] = Amp<float>{re[v], im[v]};
+                lds[j + (j >> PS)] = Amp<T>{re[v], im[v]};
             }
         }
         if (w + gridDim.x < numTiles) loadInputs(w + gridDim.x);
         __syncthreads();
         // second superoperator: LDS -> HBM
-        Amp<float> *dst = a + insertZeros((uint64_t)w, sp.tile);
+        Amp<T> *dst = a + insertZeros((uint64_t)w, sp.tile);
 #pragma unroll
         for (int i = 0; i < kMfmaBatches; ++i) {
             unsigned bt;
             uint64_t bg;
             batchBits(sp.pb, wave + i * (kBlock / 64), bt, bg);
-            Amp<float> y[4];
+            Amp<T> y[4];
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const unsigned j = bt | LB.t[s];
                 y[s] = lds[j + (j >> PS)];
             }
-            pairF4 re, im;
+            typename MfmaAcc<T>::type re, im;
             mfmaGroups(fb, y, re, im);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) dst[bg | LB.g[v]] = Amp<float>{re[v], im[v]};
+            for (int v = 0; v < 4; ++v) dst[bg | LB.g[v]] = Amp<T>{re[v], im[v]};
         }
         __syncthreads();  // the next tile's first phase overwrites the LDS copy
     }
@@ -506,9 +523,9 @@ void launchDensePair(QuregDev *q, const DensePairSpec &sp, const Cplx *MA, const
     constexpr int PS = sizeof(T) == 8 ? 3 : 4;
     constexpr int entries = 1 << kPairTileBits;
     const size_t lds = (size_t)(entries + (entries >> PS)) * sizeof(Amp<T>);
-    // FP32 4 + 4: the matrix-core form
-    constexpr bool mfma = sizeof(T) == 4 && BA == 4 && BB == 4;
-    const void *kern = mfma ? reinterpret_cast<const void *>(&kDensePairMfma)
+    // 4 + 4 bits: the matrix-core form
+    constexpr bool mfma = BA == 4 && BB == 4;
+    const void *kern = mfma ? reinterpret_cast<const void *>(&kDensePairMfma<T>)
                             : reinterpret_cast<const void *>(&kDensePair<T, BA, BB>);
     static bool ldsAllowed = false;
     if (!ldsAllowed) {
@@ -521,9 +538,10 @@ void launchDensePair(QuregDev *q, const DensePairSpec &sp, const Cplx *MA, const
     std::copy(MB, MB + (1 << (2 * BB)), both + 256);
     const Amp<T> *m = stageMatrix<T>(both, 512, st);
     const unsigned grid = (unsigned)(q->numAmps >> kPairTileBits);
-    if constexpr (mfma)  // persistent: 3 workgroups per CU (its occupancy)
-        hipLaunchKernelGGL(kDensePairMfma, dim3(std::min(grid, 3u * (unsigned)numCUs())), dim3(kBlock), lds, st,
-                           (Amp<float> *)q->amps, grid, sp, (const Amp<float> *)m, (const Amp<float> *)m + 256);
+    // persistent: FP32 3 workgroups per CU (its occupancy), FP64 2 (its LDS)
+    if constexpr (mfma)
+        hipLaunchKernelGGL(kDensePairMfma<T>, dim3(std::min(grid, (sizeof(T) == 8 ? 2u : 3u) * (unsigned)numCUs())),
+                           dim3(kBlock), lds, st, (Amp<T> *)q->amps, grid, sp, m, m + 256);
     else
         hipLaunchKernelGGL((kDensePair<T, BA, BB>), dim3(grid), dim3(kBlock), lds, st, (Amp<T> *)q->amps, sp, m,
                            m + 256);
