@@ -240,7 +240,7 @@ def run_density(qe, n, layers, seed):
     qe.destroy(pure)
     a, b = outs["fp64"], outs["fp32"]
     res.update(outs)
-    res["roofline"] = {"bound": "hbm", "kernel": "kDensePair<double, 4, 4> (two composed superoperators per sweep of the 2n-qubit vector)",
+    res["roofline"] = {"bound": "hbm", "kernel": "kDensePairMfma<double> (two composed 2-qubit superoperators per sweep of the 2n-qubit vector, on the matrix cores)",
                        "achieved": a["gb_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": round(a["gb_per_s"] / HBM_PEAK_GBS, 4), "traffic": None,
                        "note": "read + write of every entry per sweep (2 * 4^n * 16 B) over the leg's wall time; "
