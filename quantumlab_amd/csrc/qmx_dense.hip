@@ -523,10 +523,11 @@ void launchDensePair(QuregDev *q, const DensePairSpec &sp, const Cplx *MA, const
     constexpr int PS = sizeof(T) == 8 ? 3 : 4;
     constexpr int entries = 1 << kPairTileBits;
     const size_t lds = (size_t)(entries + (entries >> PS)) * sizeof(Amp<T>);
-    // 4 + 4 bits: the matrix-core form
+    // 4 + 4 bits: the matrix-core form; 4 + 2 bits: the VALU form
     constexpr bool mfma = BA == 4 && BB == 4;
-    const void *kern = mfma ? reinterpret_cast<const void *>(&kDensePairMfma<T>)
-                            : reinterpret_cast<const void *>(&kDensePair<T, BA, BB>);
+    const void *kern;
+    if constexpr (mfma) kern = reinterpret_cast<const void *>(&kDensePairMfma<T>);
+    else kern = reinterpret_cast<const void *>(&kDensePair<T, BA, BB>);
     static bool ldsAllowed = false;
     if (!ldsAllowed) {
         if (!allowDynamicLds(kern, lds, "applyDensePair")) std::abort();
@@ -550,7 +551,9 @@ void launchDensePair(QuregDev *q, const DensePairSpec &sp, const Cplx *MA, const
 }  // namespace
 
 bool densePairSupported(const QuregDev *q, int BA, int BB) {
-    return (BA == 2 || BA == 4) && (BB == 2 || BB == 4) && q->numQubits >= kPairTileBits;
+    // 4 + 4 or 4 + 2 bits (two one-qubit superoperators are never pending
+    // together: a lone one-qubit operation joins the pending one-qubit block)
+    return ((BA == 4 && (BB == 2 || BB == 4)) || (BA == 2 && BB == 4)) && q->numQubits >= kPairTileBits;
 }
 
 void applyDensePair(QuregDev *q, const int *bitsA, int BA, const Cplx *MA, const int *bitsB, int BB,
@@ -595,9 +598,8 @@ void applyDensePair(QuregDev *q, const int *bitsA, int BA, const Cplx *MA, const
     pairGlobal(sp.pb, tpos, BB);
     hipStream_t st = hstream();
     QMX_DISPATCH(q->prec, T, {
-        if (BA == 4 && BB == 4) launchDensePair<T, 4, 4>(q, sp, MA, MB, st);
-        else if (BA == 4) launchDensePair<T, 4, 2>(q, sp, MA, MB, st);
-        else launchDensePair<T, 2, 2>(q, sp, MA, MB, st);
+        if (BB == 4) launchDensePair<T, 4, 4>(q, sp, MA, MB, st);
+        else launchDensePair<T, 4, 2>(q, sp, MA, MB, st);
     });
     checkLaunch("applyDensePair");
 }

@@ -121,8 +121,8 @@ void amplitudeEncode(QuregDev *q, const double *vec, int qbits);
 // B = 2..5 arbitrary bits, one group per thread in registers (qmx_dense.hip)
 bool denseBitsSupported(int B);
 void applyDense(QuregDev *q, const int *bits, int B, const Cplx *M);
-// two matrices on disjoint bit sets (2 or 4 bits each) in one sweep; registers
-// of at least 12 qubits
+// two matrices on disjoint bit sets (4 + 4 or 4 + 2 bits) in one sweep;
+// registers of at least 12 qubits
 bool densePairSupported(const QuregDev *q, int BA, int BB);
 void applyDensePair(QuregDev *q, const int *bitsA, int BA, const Cplx *MA, const int *bitsB, int BB,
                     const Cplx *MB);
