@@ -49,6 +49,50 @@ __global__ void __launch_bounds__(kBlock) kMatrix2(Amp<T> *__restrict__ a, long 
     stA<T, NT>(a + i1, cmul2(u.m10, up, u.m11, lo));
 }
 
+// Targets 0-5, no controls: the partner of amplitude i (i ^ 2^t) is held by
+// lane l ^ 2^t of the same wave, so each thread loads and stores only its own
+// amplitude — one contiguous nontemporal stream, like an in-place scale — and
+// takes the partner through a cross-lane shuffle. Needs whole waves: n is a
+// multiple of the workgroup (registers of at least 8 qubits).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kMatrix2Lane(Amp<T> *__restrict__ a, long long n, int target, M2<T> u) {
+    const long long i = flatBlock() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Amp<T> v = ldA<T, true>(a + i);
+    const int m = 1 << target;
+    const Amp<T> p{__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64)};
+    // bit t of i clear: v is the pair's upper amplitude, p its lower one
+    const Amp<T> o = ((i >> target) & 1) ? cmul2(u.m10, p, u.m11, v) : cmul2(u.m00, v, u.m01, p);
+    stA<T, true>(a + i, o);
+}
+
+// Targets 6-10, no controls: a workgroup owns 2048 consecutive amplitudes, 8
+// per thread, loaded and stored as contiguous nontemporal streams; each
+// thread reads its amplitudes' partners from LDS after one barrier. Needs
+// registers of at least 11 qubits (whole spans).
+constexpr int kSpanPer = 8, kSpan = kBlock * kSpanPer;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kMatrix2Span(Amp<T> *__restrict__ a, int target, M2<T> u) {
+    __shared__ Amp<T> s[kSpan];
+    Amp<T> *src = a + flatBlock() * kSpan;
+    Amp<T> v[kSpanPer];
+#pragma unroll
+    for (int k = 0; k < kSpanPer; ++k) {
+        v[k] = ldA<T, true>(src + threadIdx.x + k * kBlock);
+        s[threadIdx.x + k * kBlock] = v[k];
+    }
+    __syncthreads();
+    const int m = 1 << target;
+#pragma unroll
+    for (int k = 0; k < kSpanPer; ++k) {
+        const int j = threadIdx.x + k * kBlock;
+        const Amp<T> p = s[j ^ m];
+        const Amp<T> o = ((j >> target) & 1) ? cmul2(u.m10, p, u.m11, v[k]) : cmul2(u.m00, v[k], u.m01, p);
+        stA<T, true>(src + j, o);
+    }
+}
+
 // ---- permutations ----------------------------------------------------------
 // multiControlledMultiQubitNot (QuEST_cpu.c:2761-2810), controlledNot
 // (QuEST_cpu.c:2662-2710): swap i <-> i ^ targMask where the controls are 1
@@ -61,6 +105,23 @@ __global__ void __launch_bounds__(kBlock) kNot(Amp<T> *__restrict__ a, long long
     const Amp<T> x = ldA<T, NT>(a + i), y = ldA<T, NT>(a + k);
     stA<T, NT>(a + i, y);
     stA<T, NT>(a + k, x);
+}
+
+// Target bits all below 6: each thread streams its own amplitude and, where
+// the controls are 1, takes its partner's (lane l ^ targMask of the same wave)
+// through a cross-lane shuffle. It reads and writes every amplitude once,
+// contiguously; the pair form touches only the controlled ones but, with a
+// control or target among the lowest bits, still moves whole cache lines,
+// partly written (a CNOT on qubits 0 and 1 at 32 qubits FP64: 108 GiB in
+// 24.0 ms).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kNotLane(Amp<T> *__restrict__ a, long long n, uint64_t ctrlMask,
+                                                   int targMask) {
+    const long long i = flatBlock() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Amp<T> v = ldA<T, true>(a + i);
+    const Amp<T> p{__shfl_xor(v.x, targMask, 64), __shfl_xor(v.y, targMask, 64)};
+    stA<T, true>(a + i, (((uint64_t)i & ctrlMask) == ctrlMask) ? p : v);
 }
 
 // swapGate (QuEST_cpu.c:3905-3950): |..0..1..> <-> |..1..0..>
@@ -252,7 +313,12 @@ void applyMatrix2(QuregDev *q, int target, uint64_t ctrlMask, uint64_t flipMask,
         M2<T> m{c2<T>(u[0]), c2<T>(u[1]), c2<T>(u[2]), c2<T>(u[3])};
         Amp<T> *a = (Amp<T> *)q->amps;
         const dim3 g(gridFor(work, 1));
-        if (ctrlMask) {
+        if (!ctrlMask && target < 6 && q->numQubits >= 8) {
+            hipLaunchKernelGGL(kMatrix2Lane<T>, gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), a, q->numAmps,
+                               target, m);
+        } else if (!ctrlMask && target < 11 && q->numQubits >= 11) {
+            hipLaunchKernelGGL(kMatrix2Span<T>, gridOf(q->numAmps / kSpan), dim3(kBlock), 0, hstream(), a, target, m);
+        } else if (ctrlMask) {
             if (nt) hipLaunchKernelGGL((kMatrix2<T, true, true>), g, dim3(kBlock), 0, hstream(), a, work, target, fixed, setMask, m);
             else hipLaunchKernelGGL((kMatrix2<T, true, false>), g, dim3(kBlock), 0, hstream(), a, work, target, fixed, setMask, m);
         } else {
@@ -270,7 +336,14 @@ void applyNot(QuregDev *q, uint64_t ctrlMask, uint64_t targMask) {
     BitList fixed = makeBits(ctrlMask | lowT);
     const bool nt = useNT(lowestBit(ctrlMask | targMask), q->prec);
     QMX_DISPATCH(q->prec, T, {
-        QMX_LAUNCH_NT(nt, work, kNot, (Amp<T> *)q->amps, work, fixed, ctrlMask, targMask);
+        // the lane form moves every amplitude: it wins where the pair form also
+        // moves whole lines (no controls, or a lowest bit below the
+        // nontemporal threshold) and loses where the controls halve the traffic
+        if (targMask < 64 && q->numQubits >= 8 && (!ctrlMask || !nt))
+            hipLaunchKernelGGL(kNotLane<T>, gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), (Amp<T> *)q->amps,
+                               q->numAmps, ctrlMask, (int)targMask);
+        else
+            QMX_LAUNCH_NT(nt, work, kNot, (Amp<T> *)q->amps, work, fixed, ctrlMask, targMask);
     });
     checkLaunch("applyNot");
 }
