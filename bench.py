@@ -554,7 +554,7 @@ def statevec_leg(qe, args, n, N):
                                f"5-qubit multiQubitUnitary, per-gate QuEST C-API kernels",
                    "qubits": n, "gates_per_step": len(timed) // args.steps, "seed": args.seed},
         "gates_per_s": round(len(timed) / wall, 2),
-        "roofline": {"bound": "hbm", "kernel": "kMatrix2<double> (compactUnitary 2x2 pair sweep)",
+        "roofline": {"bound": "hbm", "kernel": "compactUnitary 2x2 sweep: kMatrix2Lane<double> (targets 0-5), kMatrix2Span<double> (6-10), kMatrix2<double> (11-31)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(float(cu_ms.mean()), 4), "launches": int(len(cu_ms)),
