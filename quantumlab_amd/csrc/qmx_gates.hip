@@ -49,20 +49,24 @@ __global__ void __launch_bounds__(kBlock) kMatrix2(Amp<T> *__restrict__ a, long 
     stA<T, NT>(a + i1, cmul2(u.m10, up, u.m11, lo));
 }
 
-// Targets 0-5, no controls: the partner of amplitude i (i ^ 2^t) is held by
-// lane l ^ 2^t of the same wave, so each thread loads and stores only its own
-// amplitude — one contiguous nontemporal stream, like an in-place scale — and
-// takes the partner through a cross-lane shuffle. Needs whole waves: n is a
-// multiple of the workgroup (registers of at least 8 qubits).
-template <typename T>
-__global__ void __launch_bounds__(kBlock) kMatrix2Lane(Amp<T> *__restrict__ a, long long n, int target, M2<T> u) {
+// Targets 0-5: the partner of amplitude i (i ^ 2^t) is held by lane l ^ 2^t
+// of the same wave, so each thread loads and stores only its own amplitude —
+// one contiguous nontemporal stream, like an in-place scale — and takes the
+// partner through a cross-lane shuffle. With controls (CTRL), amplitudes whose
+// control bits differ from setMask are written back unchanged: used where the
+// pair form, touching only the controlled ones, would still move whole lines.
+// Needs whole waves: n is a multiple of the workgroup (at least 8 qubits).
+template <typename T, bool CTRL>
+__global__ void __launch_bounds__(kBlock) kMatrix2Lane(Amp<T> *__restrict__ a, long long n, int target,
+                                                       uint64_t ctrlMask, uint64_t setMask, M2<T> u) {
     const long long i = flatBlock() * kBlock + threadIdx.x;
     if (i >= n) return;
     const Amp<T> v = ldA<T, true>(a + i);
     const int m = 1 << target;
     const Amp<T> p{__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64)};
     // bit t of i clear: v is the pair's upper amplitude, p its lower one
-    const Amp<T> o = ((i >> target) & 1) ? cmul2(u.m10, p, u.m11, v) : cmul2(u.m00, v, u.m01, p);
+    Amp<T> o = ((i >> target) & 1) ? cmul2(u.m10, p, u.m11, v) : cmul2(u.m00, v, u.m01, p);
+    if (CTRL && ((uint64_t)i & ctrlMask) != setMask) o = v;
     stA<T, true>(a + i, o);
 }
 
@@ -314,8 +318,13 @@ void applyMatrix2(QuregDev *q, int target, uint64_t ctrlMask, uint64_t flipMask,
         Amp<T> *a = (Amp<T> *)q->amps;
         const dim3 g(gridFor(work, 1));
         if (!ctrlMask && target < 6 && q->numQubits >= 8) {
-            hipLaunchKernelGGL(kMatrix2Lane<T>, gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), a, q->numAmps,
-                               target, m);
+            hipLaunchKernelGGL((kMatrix2Lane<T, false>), gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), a,
+                               q->numAmps, target, 0ull, 0ull, m);
+        } else if (ctrlMask && !useNT(lowestBit(ctrlMask), q->prec) && target < 6 && q->numQubits >= 8) {
+            // a control among the lowest bits: the pair form would move whole
+            // lines of which it changes half
+            hipLaunchKernelGGL((kMatrix2Lane<T, true>), gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), a,
+                               q->numAmps, target, ctrlMask, setMask, m);
         } else if (!ctrlMask && target < 11 && q->numQubits >= 11) {
             hipLaunchKernelGGL(kMatrix2Span<T>, gridOf(q->numAmps / kSpan), dim3(kBlock), 0, hstream(), a, target, m);
         } else if (ctrlMask) {
@@ -337,9 +346,10 @@ void applyNot(QuregDev *q, uint64_t ctrlMask, uint64_t targMask) {
     const bool nt = useNT(lowestBit(ctrlMask | targMask), q->prec);
     QMX_DISPATCH(q->prec, T, {
         // the lane form moves every amplitude: it wins where the pair form also
-        // moves whole lines (no controls, or a lowest bit below the
-        // nontemporal threshold) and loses where the controls halve the traffic
-        if (targMask < 64 && q->numQubits >= 8 && (!ctrlMask || !nt))
+        // moves whole lines (no controls, or a control among the lowest bits,
+        // so that every line holds controlled and uncontrolled amplitudes) and
+        // loses where high controls halve the traffic
+        if (targMask < 64 && q->numQubits >= 8 && (!ctrlMask || !useNT(lowestBit(ctrlMask), q->prec)))
             hipLaunchKernelGGL(kNotLane<T>, gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), (Amp<T> *)q->amps,
                                q->numAmps, ctrlMask, (int)targMask);
         else
