@@ -10,7 +10,7 @@ import pytest
 from quantumlab_amd import workloads as W
 from quantumlab_amd.quest import QMX_PREC_FP32, QuESTError, Vector, cplx, matrix2
 from tests import dense as D
-from tests.cases import channel_circuit, misc_ops, pauli_rotation_cases, random_hamil, random_state
+from tests.cases import channel_circuit, ints, misc_ops, pauli_rotation_cases, random_hamil, random_state
 from tests.conftest import golden
 
 pytestmark = pytest.mark.gpu
@@ -389,3 +389,32 @@ def test_large_circuit_round_trip_returns_to_zero_state(gpu, n):
     assert abs(a0.real - 1) < 1e-12 and abs(a0.imag) < 1e-12
     assert abs(gpu.calcProbOfOutcome(q, n - 1, 0) - 1) < 1e-12
     gpu.destroy(q)
+
+
+def test_low_target_stream_forms(gpu, cpuref):
+    """2x2 gates and NOTs on the lowest qubits run as in-place streams: partners
+    by cross-lane shuffle (targets 0-5) or through LDS (targets 6-10), with
+    controls where a control is among the lowest bits (amplitudes whose controls
+    differ written back unchanged), including controls on |0>. Every target on
+    a 12-qubit register, against the oracle."""
+    n = 12
+    rng = np.random.default_rng(77)
+
+    def ops(lib, q):
+        for t in range(n):
+            a, b = pairs[t]
+            lib.call("compactUnitary", q, t, cplx(a), cplx(b))
+            for c in (0, 1, 2, 7, 11):
+                if c != t:
+                    lib.call("controlledCompactUnitary", q, c, t, cplx(b), cplx(a))
+                    lib.call("controlledNot", q, c, t)
+            lib.call("pauliX", q, t)
+            lib.call("multiStateControlledUnitary", q, ints(ctrls[t]), ints([0, 1]), 2, t, matrix2(us[t]))
+        lib.call("multiQubitNot", q, ints([0, 3, 5]), 3)
+        lib.call("multiControlledMultiQubitNot", q, ints([1, 9]), 2, ints([0, 4]), 2)
+
+    pairs = [W.compact_pair(rng) for _ in range(n)]
+    us = [W.haar_unitary(rng, 2) for _ in range(n)]
+    ctrls = [[c for c in (0, 1, 2, 3) if c != t][:2] for t in range(n)]
+    a, b = run_both(gpu, cpuref, n, ops, init=lambda lib, q: lib.set_state(q, random_state(n, 9)))
+    assert np.max(np.abs(a - b)) < TOL
