@@ -141,6 +141,19 @@ __global__ void __launch_bounds__(kBlock) kSwap(Amp<T> *__restrict__ a, long lon
     stA<T, NT>(a + k, x);
 }
 
+// swapGate with both qubits below 6 and one among the lowest bits: every
+// amplitude streamed, exchanged with lane l ^ (m1 | m2) where the two bits
+// differ (the pair form would move whole lines of which it changes half)
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kSwapLane(Amp<T> *__restrict__ a, long long n, int m1, int m2) {
+    const long long i = flatBlock() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Amp<T> v = ldA<T, true>(a + i);
+    const Amp<T> p{__shfl_xor(v.x, m1 | m2, 64), __shfl_xor(v.y, m1 | m2, 64)};
+    const bool differ = (((int)i & m1) != 0) != (((int)i & m2) != 0);
+    stA<T, true>(a + i, differ ? p : v);
+}
+
 // ---- diagonal gates --------------------------------------------------------
 // phaseShiftByTerm / multiControlledPhaseShift / phase flips: amp *= term on
 // the 2^(n-m) amplitudes whose mask bits are all 1
@@ -364,7 +377,11 @@ void applySwap(QuregDev *q, int q1, int q2) {
     BitList fixed = makeBits(m1 | m2);
     const bool nt = useNT(lowestBit(m1 | m2), q->prec);
     QMX_DISPATCH(q->prec, T, {
-        QMX_LAUNCH_NT(nt, work, kSwap, (Amp<T> *)q->amps, work, fixed, m1, m2);
+        if ((m1 | m2) < 64 && !nt && q->numQubits >= 8)
+            hipLaunchKernelGGL(kSwapLane<T>, gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), (Amp<T> *)q->amps,
+                               q->numAmps, (int)m1, (int)m2);
+        else
+            QMX_LAUNCH_NT(nt, work, kSwap, (Amp<T> *)q->amps, work, fixed, m1, m2);
     });
     checkLaunch("applySwap");
 }

@@ -395,8 +395,8 @@ def test_low_target_stream_forms(gpu, cpuref):
     """2x2 gates and NOTs on the lowest qubits run as in-place streams: partners
     by cross-lane shuffle (targets 0-5) or through LDS (targets 6-10), with
     controls where a control is among the lowest bits (amplitudes whose controls
-    differ written back unchanged), including controls on |0>. Every target on
-    a 12-qubit register, against the oracle."""
+    differ written back unchanged), including controls on |0>, and swaps of two
+    low qubits. Every target on a 12-qubit register, against the oracle."""
     n = 12
     rng = np.random.default_rng(77)
 
@@ -410,6 +410,8 @@ def test_low_target_stream_forms(gpu, cpuref):
                     lib.call("controlledNot", q, c, t)
             lib.call("pauliX", q, t)
             lib.call("multiStateControlledUnitary", q, ints(ctrls[t]), ints([0, 1]), 2, t, matrix2(us[t]))
+        for q1, q2 in ((0, 1), (2, 5), (1, 4), (0, 9), (3, 11)):
+            lib.call("swapGate", q, q1, q2)
         lib.call("multiQubitNot", q, ints([0, 3, 5]), 3)
         lib.call("multiControlledMultiQubitNot", q, ints([1, 9]), 2, ints([0, 4]), 2)
 
