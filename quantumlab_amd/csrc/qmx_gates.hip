@@ -202,6 +202,25 @@ __global__ void __launch_bounds__(kBlock) kPauliRot(Amp<T> *__restrict__ a, long
     stA<T, NT>(a + p, Amp<T>{c * vp.x + sp * (kre * vi.x - kim * vi.y), c * vp.y + sp * (kre * vi.y + kim * vi.x)});
 }
 
+// The same rotation with every X/Y qubit below 6: each thread streams its own
+// amplitude and takes its partner (i ^ xMask) from lane l ^ xMask; with
+// controls among the lowest bits (CTRL), uncontrolled amplitudes are written
+// back unchanged. Output i uses the sign of parity((i ^ xMask) & zMask), as
+// the pair form's out_i does.
+template <typename T, bool CTRL>
+__global__ void __launch_bounds__(kBlock) kPauliRotLane(Amp<T> *__restrict__ a, long long n, uint64_t ctrlMask,
+                                                        int xMask, uint64_t zMask, C2<T> ph, T c, T s) {
+    const long long i = flatBlock() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const T kre = s * ph.im, kim = -s * ph.re;
+    const Amp<T> v = ldA<T, true>(a + i);
+    const Amp<T> p{__shfl_xor(v.x, xMask, 64), __shfl_xor(v.y, xMask, 64)};
+    const T si = parity64(((uint64_t)i ^ (uint64_t)xMask) & zMask) ? (T)-1 : (T)1;
+    Amp<T> o{c * v.x + si * (kre * p.x - kim * p.y), c * v.y + si * (kre * p.y + kim * p.x)};
+    if (CTRL && ((uint64_t)i & ctrlMask) != ctrlMask) o = v;
+    stA<T, true>(a + i, o);
+}
+
 // ---- dense 2^k x 2^k on k targets: LDS-tiled --------------------------------
 // QuEST_cpu.c:1895-2007 semantics (targs[0] = least significant bit of the
 // matrix row/column index). A workgroup owns a tile of 2^(k+L) amplitudes:
@@ -425,9 +444,19 @@ void applyPauliRotation(QuregDev *q, uint64_t ctrlMask, uint64_t xMask, uint64_t
     static const double phr[4] = {1, 0, -1, 0}, phi[4] = {0, 1, 0, -1};
     Cplx ph{phr[numY & 3], phi[numY & 3]};
     const bool nt = useNT(lowestBit(ctrlMask | xMask), q->prec);
+    // X/Y qubits below 6 and no control, or one among the lowest bits: the
+    // in-place stream (the pair form would move whole lines anyway)
+    const bool lane = xMask < 64 && q->numQubits >= 8 && (!ctrlMask || !useNT(lowestBit(ctrlMask), q->prec));
     QMX_DISPATCH(q->prec, T, {
-        QMX_LAUNCH_NT(nt, work, kPauliRot, (Amp<T> *)q->amps, work, fixed, ctrlMask, xMask, zMask, c2<T>(ph),
-                      (T)cs, (T)sn);
+        if (lane && ctrlMask)
+            hipLaunchKernelGGL((kPauliRotLane<T, true>), gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(),
+                               (Amp<T> *)q->amps, q->numAmps, ctrlMask, (int)xMask, zMask, c2<T>(ph), (T)cs, (T)sn);
+        else if (lane)
+            hipLaunchKernelGGL((kPauliRotLane<T, false>), gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(),
+                               (Amp<T> *)q->amps, q->numAmps, 0ull, (int)xMask, zMask, c2<T>(ph), (T)cs, (T)sn);
+        else
+            QMX_LAUNCH_NT(nt, work, kPauliRot, (Amp<T> *)q->amps, work, fixed, ctrlMask, xMask, zMask, c2<T>(ph),
+                          (T)cs, (T)sn);
     });
     checkLaunch("applyPauliRotation");
 }

@@ -395,8 +395,9 @@ def test_low_target_stream_forms(gpu, cpuref):
     """2x2 gates and NOTs on the lowest qubits run as in-place streams: partners
     by cross-lane shuffle (targets 0-5) or through LDS (targets 6-10), with
     controls where a control is among the lowest bits (amplitudes whose controls
-    differ written back unchanged), including controls on |0>, and swaps of two
-    low qubits. Every target on a 12-qubit register, against the oracle."""
+    differ written back unchanged), including controls on |0>, swaps of two
+    low qubits and Pauli-string rotations. Every target on a 12-qubit register,
+    against the oracle."""
     n = 12
     rng = np.random.default_rng(77)
 
@@ -412,6 +413,9 @@ def test_low_target_stream_forms(gpu, cpuref):
             lib.call("multiStateControlledUnitary", q, ints(ctrls[t]), ints([0, 1]), 2, t, matrix2(us[t]))
         for q1, q2 in ((0, 1), (2, 5), (1, 4), (0, 9), (3, 11)):
             lib.call("swapGate", q, q1, q2)
+        for targs, paulis, ang in (([0, 1], [1, 2], 0.3), ([2, 5, 9], [2, 3, 1], -0.7), ([4], [1], 1.1),
+                                   ([0, 3, 7, 11], [3, 2, 2, 1], 0.5)):
+            lib.multiRotatePauli(q, targs, paulis, ang)
         lib.call("multiQubitNot", q, ints([0, 3, 5]), 3)
         lib.call("multiControlledMultiQubitNot", q, ints([1, 9]), 2, ints([0, 4]), 2)
 
@@ -420,3 +424,24 @@ def test_low_target_stream_forms(gpu, cpuref):
     ctrls = [[c for c in (0, 1, 2, 3) if c != t][:2] for t in range(n)]
     a, b = run_both(gpu, cpuref, n, ops, init=lambda lib, q: lib.set_state(q, random_state(n, 9)))
     assert np.max(np.abs(a - b)) < TOL
+
+
+def test_controlled_pauli_rotations_match_dense(gpu):
+    """multiControlledMultiRotatePauli with low controls (in-place stream form)
+    and high ones (pair form) against the dense operator exp(-i theta/2 P) on
+    the controlled subspace, 10 qubits."""
+    n = 10
+    psi = random_state(n, 12)
+    q = gpu.qureg(n)
+    gpu.set_state(q, psi)
+    want = psi.copy()
+    for cs, targs, paulis, ang in (([0], [2, 4], [1, 2], 0.9), ([9], [1, 3], [2, 1], -0.4),
+                                   ([1, 6], [0, 5], [1, 3], 0.2), ([2], [0], [2], 1.3), ([5, 0], [3, 8], [1, 1], 0.7)):
+        gpu.call("multiControlledMultiRotatePauli", q, ints(cs), len(cs), ints(targs), ints(paulis), len(targs), ang)
+        P = np.ones((1, 1), dtype=complex)
+        for p in paulis:                      # targets[0] is the least significant factor
+            P = np.kron(D.PAULI[p], P)
+        U = np.cos(ang / 2) * np.eye(len(P)) - 1j * np.sin(ang / 2) * P
+        want = D.full_op(n, targs, U, controls=cs) @ want
+    assert np.max(np.abs(gpu.state(q) - want)) < TOL
+    gpu.destroy(q)
