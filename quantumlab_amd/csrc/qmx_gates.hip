@@ -326,6 +326,15 @@ size_t g_matBytes = 0;
 
 int lowestBit(uint64_t m) { return m ? __builtin_ctzll(m) : 63; }
 
+// Every control among the lowest bits (below the nontemporal threshold): each
+// cache line then holds every control combination, so a pair form that
+// touches only the controlled amplitudes still moves every line. With a
+// control above it, the pair form moves only the lines that control selects
+// and beats an in-place stream of the whole register.
+bool controlsAllLow(uint64_t ctrlMask, int prec) {
+    return ctrlMask && !useNT(63 - __builtin_clzll(ctrlMask), prec);
+}
+
 #define QMX_LAUNCH_NT(nt, work, KERN, ...)                                                            \
     do {                                                                                             \
         if (nt)                                                                                      \
@@ -352,9 +361,9 @@ void applyMatrix2(QuregDev *q, int target, uint64_t ctrlMask, uint64_t flipMask,
         if (!ctrlMask && target < 6 && q->numQubits >= 8) {
             hipLaunchKernelGGL((kMatrix2Lane<T, false>), gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), a,
                                q->numAmps, target, 0ull, 0ull, m);
-        } else if (ctrlMask && !useNT(lowestBit(ctrlMask), q->prec) && target < 6 && q->numQubits >= 8) {
-            // a control among the lowest bits: the pair form would move whole
-            // lines of which it changes half
+        } else if (controlsAllLow(ctrlMask, q->prec) && target < 6 && q->numQubits >= 8) {
+            // all controls among the lowest bits: the pair form would move
+            // every line while changing only part of each
             hipLaunchKernelGGL((kMatrix2Lane<T, true>), gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), a,
                                q->numAmps, target, ctrlMask, setMask, m);
         } else if (!ctrlMask && target < 11 && q->numQubits >= 11) {
@@ -378,10 +387,9 @@ void applyNot(QuregDev *q, uint64_t ctrlMask, uint64_t targMask) {
     const bool nt = useNT(lowestBit(ctrlMask | targMask), q->prec);
     QMX_DISPATCH(q->prec, T, {
         // the lane form moves every amplitude: it wins where the pair form also
-        // moves whole lines (no controls, or a control among the lowest bits,
-        // so that every line holds controlled and uncontrolled amplitudes) and
-        // loses where high controls halve the traffic
-        if (targMask < 64 && q->numQubits >= 8 && (!ctrlMask || !useNT(lowestBit(ctrlMask), q->prec)))
+        // moves every line (no controls, or all of them among the lowest bits)
+        // and loses where a high control halves the traffic
+        if (targMask < 64 && q->numQubits >= 8 && (!ctrlMask || controlsAllLow(ctrlMask, q->prec)))
             hipLaunchKernelGGL(kNotLane<T>, gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), (Amp<T> *)q->amps,
                                q->numAmps, ctrlMask, (int)targMask);
         else
@@ -444,9 +452,9 @@ void applyPauliRotation(QuregDev *q, uint64_t ctrlMask, uint64_t xMask, uint64_t
     static const double phr[4] = {1, 0, -1, 0}, phi[4] = {0, 1, 0, -1};
     Cplx ph{phr[numY & 3], phi[numY & 3]};
     const bool nt = useNT(lowestBit(ctrlMask | xMask), q->prec);
-    // X/Y qubits below 6 and no control, or one among the lowest bits: the
-    // in-place stream (the pair form would move whole lines anyway)
-    const bool lane = xMask < 64 && q->numQubits >= 8 && (!ctrlMask || !useNT(lowestBit(ctrlMask), q->prec));
+    // X/Y qubits below 6 and no control, or all of them among the lowest
+    // bits: the in-place stream (the pair form would move every line anyway)
+    const bool lane = xMask < 64 && q->numQubits >= 8 && (!ctrlMask || controlsAllLow(ctrlMask, q->prec));
     QMX_DISPATCH(q->prec, T, {
         if (lane && ctrlMask)
             hipLaunchKernelGGL((kPauliRotLane<T, true>), gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(),
