@@ -167,6 +167,19 @@ __global__ void __launch_bounds__(kBlock) kPhaseAllOnes(Amp<T> *__restrict__ a, 
     stA<T, NT>(a + i, Amp<T>{t.re * v.x - t.im * v.y, t.re * v.y + t.im * v.x});
 }
 
+// The same with every mask bit among the lowest bits: each line then holds
+// amplitudes inside and outside the mask, so the direct form moves every line;
+// the in-place stream multiplies the matching amplitudes and writes the others
+// back unchanged.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kPhaseLane(Amp<T> *__restrict__ a, long long n, uint64_t mask, C2<T> t) {
+    const long long i = flatBlock() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Amp<T> v = ldA<T, true>(a + i);
+    const bool hit = ((uint64_t)i & mask) == mask;
+    stA<T, true>(a + i, hit ? Amp<T>{t.re * v.x - t.im * v.y, t.re * v.y + t.im * v.x} : v);
+}
+
 // multiRotateZ (QuEST_cpu.c:3299-3339): amp *= cos(θ/2) - i·f·sin(θ/2),
 // f = +1 / -1 for even / odd parity of (i & targMask); only where ctrls are 1
 template <typename T, bool NT>
@@ -419,7 +432,11 @@ void applyPhaseOnAllOnes(QuregDev *q, uint64_t mask, Cplx term) {
     BitList fixed = makeBits(mask);
     const bool nt = useNT(lowestBit(mask), q->prec);
     QMX_DISPATCH(q->prec, T, {
-        QMX_LAUNCH_NT(nt, work, kPhaseAllOnes, (Amp<T> *)q->amps, work, fixed, mask, c2<T>(term));
+        if (controlsAllLow(mask, q->prec) && q->numQubits >= 8)
+            hipLaunchKernelGGL(kPhaseLane<T>, gridFor(q->numAmps, 1), dim3(kBlock), 0, hstream(), (Amp<T> *)q->amps,
+                               q->numAmps, mask, c2<T>(term));
+        else
+            QMX_LAUNCH_NT(nt, work, kPhaseAllOnes, (Amp<T> *)q->amps, work, fixed, mask, c2<T>(term));
     });
     checkLaunch("applyPhaseOnAllOnes");
 }

@@ -396,8 +396,8 @@ def test_low_target_stream_forms(gpu, cpuref):
     by cross-lane shuffle (targets 0-5) or through LDS (targets 6-10), with
     controls where a control is among the lowest bits (amplitudes whose controls
     differ written back unchanged), including controls on |0>, swaps of two
-    low qubits and Pauli-string rotations. Every target on a 12-qubit register,
-    against the oracle."""
+    low qubits, Pauli-string rotations and phase gates on the lowest qubits.
+    Every target on a 12-qubit register, against the oracle."""
     n = 12
     rng = np.random.default_rng(77)
 
@@ -416,6 +416,13 @@ def test_low_target_stream_forms(gpu, cpuref):
         for targs, paulis, ang in (([0, 1], [1, 2], 0.3), ([2, 5, 9], [2, 3, 1], -0.7), ([4], [1], 1.1),
                                    ([0, 3, 7, 11], [3, 2, 2, 1], 0.5)):
             lib.multiRotatePauli(q, targs, paulis, ang)
+        lib.call("phaseShift", q, 0, 0.4)
+        lib.call("sGate", q, 1)
+        lib.call("tGate", q, 2)
+        lib.call("controlledPhaseShift", q, 0, 2, -0.8)
+        lib.call("controlledPhaseFlip", q, 1, 2)
+        lib.call("controlledPhaseShift", q, 1, 9, 0.6)
+        lib.call("multiControlledPhaseFlip", q, ints([0, 1, 2]), 3)
         lib.call("multiQubitNot", q, ints([0, 3, 5]), 3)
         lib.call("multiControlledMultiQubitNot", q, ints([1, 9]), 2, ints([0, 4]), 2)
 
