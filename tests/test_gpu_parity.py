@@ -360,6 +360,46 @@ def test_density_validation_messages(gpu):
     gpu.destroy(rho)
 
 
+def test_mismatched_register_messages(gpu):
+    """QuEST340/tests/test_calculations.cpp:93,270-285,687-692,859-868,1298,1410
+    and test_data_structures.cpp:158-165,638-644: register pairs of different
+    sizes or kinds, Hamiltonians of the wrong width, and registers too large
+    for a size_t amplitude count are refused with QuEST's messages."""
+    v3, v4 = gpu.qureg(3), gpu.qureg(4)
+    m3, m4 = gpu.qureg(3, density=True), gpu.qureg(4, density=True)
+    h3 = gpu.hamil([0.5], np.array([[1, 0, 3]]))
+    h4 = gpu.hamil([0.5], np.array([[1, 0, 3, 2]]))
+    w3, wm3 = gpu.qureg(3), gpu.qureg(3, density=True)
+    cases = [
+        (lambda: gpu.calcInnerProduct(v3, v4), "Dimensions.*don't match"),
+        (lambda: gpu.calcInnerProduct(v3, m3), "valid only for state-vectors"),
+        (lambda: gpu.calcInnerProduct(m3, v3), "valid only for state-vectors"),
+        (lambda: gpu.calcInnerProduct(m3, m3), "valid only for state-vectors"),
+        (lambda: gpu.calcFidelity(v4, v3), "Dimensions.*don't match"),
+        (lambda: gpu.calcFidelity(m4, v3), "Dimensions.*don't match"),
+        (lambda: gpu.calcDensityInnerProduct(m3, m4), "Dimensions.*don't match"),
+        (lambda: gpu.calcHilbertSchmidtDistance(m3, m4), "Dimensions.*don't match"),
+        (lambda: gpu.calcExpecPauliHamil(v3, h3, v4), "Dimensions.*don't match"),
+        (lambda: gpu.calcExpecPauliHamil(m3, h3, m4), "Dimensions.*don't match"),
+        (lambda: gpu.calcExpecPauliHamil(v3, h4, w3), "same number of qubits"),
+        (lambda: gpu.calcExpecPauliHamil(m3, h4, wm3), "same number of qubits"),
+        (lambda: gpu.getAmp(m3, 0), "valid only for state-vectors"),
+        (lambda: gpu.getNumAmps(m3), "valid only for state-vectors"),
+        (lambda: gpu.qureg(64), "Too many qubits.*size_t type"),
+        (lambda: gpu.qureg(32, density=True), "Too many qubits.*size_t type"),
+    ]
+    before = [gpu.state(q) for q in (v3, v4, m3, m4)]
+    for fn, substr in cases:
+        with pytest.raises(QuESTError, match=substr):
+            fn()
+    for q, b in zip((v3, v4, m3, m4), before):
+        assert np.array_equal(gpu.state(q), b)
+    for h in (h3, h4):
+        gpu.lib.destroyPauliHamil(h)
+    for q in (v3, v4, m3, m4, w3, wm3):
+        gpu.destroy(q)
+
+
 # ---- size-independent properties at larger sizes -----------------------------------------------
 
 def inverse(gates):
