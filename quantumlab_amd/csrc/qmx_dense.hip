@@ -152,6 +152,16 @@ struct DensePairSpec {
     PairPhase pa, pb;
 };
 
+// global index of a tile's entry 0: the tile index with zeros inserted at the
+// tile's kPairTileBits positions. The count is a constant, so the loop unrolls
+// and the positions stay in scalar registers (a loop over a run-time count
+// re-reads each position from the kernel arguments).
+__device__ __forceinline__ uint64_t pairTileBase(uint64_t tile, const BitList &b) {
+#pragma unroll
+    for (int j = 0; j < kPairTileBits; ++j) tile = insertZero(tile, b.pos[j]);
+    return tile;
+}
+
 template <typename T, int BX, bool IN_GLOBAL>
 __device__ __forceinline__ void pairPhase(Amp<T> *__restrict__ a, uint64_t tileBase, Amp<T> *lds, const PairPhase &ph,
                                           const Amp<T> *__restrict__ M) {
@@ -337,7 +347,7 @@ __global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<T> *__restrict__ a,
     const MfmaLayout LA = mfmaLayout<T>(sp.pa, col, q, true), LB = mfmaLayout<T>(sp.pb, col, q, false);
     Amp<T> x[kMfmaBatches][4];
     auto loadInputs = [&](unsigned tile) {
-        const Amp<T> *src = a + insertZeros((uint64_t)tile, sp.tile);
+        const Amp<T> *src = a + pairTileBase((uint64_t)tile, sp.tile);
 #pragma unroll
         for (int i = 0; i < kMfmaBatches; ++i) {
             unsigned bt;
@@ -367,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<T> *__restrict__ a,
         if (w + gridDim.x < numTiles) loadInputs(w + gridDim.x);
         __syncthreads();
         // second superoperator: LDS -> HBM
-        Amp<T> *dst = a + insertZeros((uint64_t)w, sp.tile);
+        Amp<T> *dst = a + pairTileBase((uint64_t)w, sp.tile);
 #pragma unroll
         for (int i = 0; i < kMfmaBatches; ++i) {
             unsigned bt;
