@@ -43,6 +43,14 @@ struct Fused5Spec {
 
 constexpr int kChunksPerWave = 4;  // 16-group chunks handled by one wave
 
+// index of group g's combination 0: g with zeros inserted at the 5 target
+// positions (a constant count, so the loop unrolls over scalar positions)
+__device__ __forceinline__ uint64_t fusedGroupBase(uint64_t g, const BitList &t) {
+#pragma unroll
+    for (int j = 0; j < 5; ++j) g = insertZero(g, t.pos[j]);
+    return g;
+}
+
 __global__ void __launch_bounds__(kBlock) kFused5(Amp<float> *__restrict__ a, long long numChunks,
                                                   Fused5Spec spec, const float *__restrict__ M) {
     const int lane = threadIdx.x & 63;
@@ -58,13 +66,21 @@ __global__ void __launch_bounds__(kBlock) kFused5(Amp<float> *__restrict__ a, lo
     __syncthreads();
 
     const long long firstChunk = (xcdBlock() * (kBlock / 64) + wave) * kChunksPerWave;
+    // Combination offsets split into a per-lane part and a wave-uniform part:
+    // input c = 4 s + kq has off[c] = off[kq] | off[4 s], output c = 16 h +
+    // 4 kq + v has off[c] = off[4 kq] | off[16 h + v] (disjoint bits). The
+    // uniform parts stay in scalar registers, so no lane reads the offset
+    // table from memory: such reads queue behind the prefetched amplitudes,
+    // and waiting for them would wait for the prefetch too.
+    const uint64_t inLane = kq == 0 ? 0 : kq == 1 ? spec.off[1] : kq == 2 ? spec.off[2] : spec.off[3];
+    const uint64_t outLane = kq == 0 ? 0 : kq == 1 ? spec.off[4] : kq == 2 ? spec.off[8] : spec.off[12];
     // the next chunk's amplitudes are loaded while the current chunk multiplies
     Amp<float> x[8], xn[8];
     uint64_t base = 0, nbase = 0;
     if (firstChunk < numChunks) {
-        nbase = insertZeros((uint64_t)firstChunk * 16 + col, spec.targetsSorted);
+        nbase = fusedGroupBase((uint64_t)firstChunk * 16 + col, spec.targetsSorted);
 #pragma unroll
-        for (int s = 0; s < 8; ++s) xn[s] = ldA<float, false>(a + (nbase + spec.off[4 * s + kq]));
+        for (int s = 0; s < 8; ++s) xn[s] = ldA<float, false>(a + ((nbase | inLane) + spec.off[4 * s]));
     }
     for (int it = 0; it < kChunksPerWave; ++it) {
         const long long chunk = firstChunk + it;
@@ -73,9 +89,9 @@ __global__ void __launch_bounds__(kBlock) kFused5(Amp<float> *__restrict__ a, lo
 #pragma unroll
         for (int s = 0; s < 8; ++s) x[s] = xn[s];
         if (it + 1 < kChunksPerWave && chunk + 1 < numChunks) {
-            nbase = insertZeros((uint64_t)(chunk + 1) * 16 + col, spec.targetsSorted);
+            nbase = fusedGroupBase((uint64_t)(chunk + 1) * 16 + col, spec.targetsSorted);
 #pragma unroll
-            for (int s = 0; s < 8; ++s) xn[s] = ldA<float, false>(a + (nbase + spec.off[4 * s + kq]));
+            for (int s = 0; s < 8; ++s) xn[s] = ldA<float, false>(a + ((nbase | inLane) + spec.off[4 * s]));
         }
         f4 acc[4];
 #pragma unroll
@@ -101,8 +117,8 @@ __global__ void __launch_bounds__(kBlock) kFused5(Amp<float> *__restrict__ a, lo
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int c = 16 * h + 4 * kq + v;
-                stA<float, false>(a + (base + spec.off[c]), Amp<float>{acc[h][v], acc[h + 2][v]});
+                stA<float, false>(a + ((base | outLane) + spec.off[16 * h + v]),
+                                  Amp<float>{acc[h][v], acc[h + 2][v]});
             }
     }
 }
