@@ -336,6 +336,18 @@ __device__ __forceinline__ void cosetDiagonal(CosetRegs<T, E> &x, T c, T kap, un
     }
 }
 
+// Tile base index: w with zeros inserted at the tile's positions. Coset
+// tiles always hold 8 + R bits, a constant count, so the loop unrolls over
+// scalar positions (a loop over a run-time count re-reads each position from
+// the kernel arguments, and the compiler then waits for every outstanding
+// store before the prefetched tile is written to LDS).
+template <int TB>
+__device__ __forceinline__ uint64_t cosetTileBase(uint64_t w, const BitList &b) {
+#pragma unroll
+    for (int j = 0; j < TB; ++j) w = insertZero(w, b.pos[j]);
+    return w;
+}
+
 // Tile staging, prefetch and the final scaled store as in kPauliBlock; the
 // groups replace its passes.
 template <typename T, bool PREFETCH>
@@ -353,19 +365,24 @@ __global__ void __launch_bounds__(kBlock) kPauliCoset(Amp<T> *__restrict__ a, un
         if ((tx >> j) & 1) dt |= 1ull << tile.pos[j];
     Amp<T> next[PREFETCH ? E : 1];
     unsigned w = blockIdx.x;
-    if (PREFETCH) {
-        const Amp<T> *ns = a + (insertZeros((uint64_t)w, tile.fixed) | dt);
+    if (PREFETCH && w < numTiles) {
+        const Amp<T> *ns = a + (cosetTileBase<8 + R>((uint64_t)w, tile.fixed) | dt);
 #pragma unroll
-        for (int k = 0; k < E; ++k) next[k] = ns[tile.hi[k]];
+        for (int k = 0; k < E; ++k) st[tx + (k << 8)] = ns[tile.hi[k]];
     }
     for (; w < numTiles; w += PREFETCH ? gridDim.x : numTiles) {
-        const uint64_t base = insertZeros((uint64_t)w, tile.fixed);
+        // one tile per workgroup without prefetch: the loop form costs one
+        // register less there
+        const uint64_t base = PREFETCH ? cosetTileBase<8 + R>((uint64_t)w, tile.fixed)
+                                       : insertZeros((uint64_t)w, tile.fixed);
         Amp<T> *src = a + (base | dt);
+        if (!PREFETCH) {
 #pragma unroll
-        for (int k = 0; k < E; ++k) st[tx + (k << 8)] = PREFETCH ? next[k] : src[tile.hi[k]];
+            for (int k = 0; k < E; ++k) st[tx + (k << 8)] = src[tile.hi[k]];
+        }
         __syncthreads();
         if (PREFETCH && w + gridDim.x < numTiles) {
-            const Amp<T> *ns = a + (insertZeros((uint64_t)(w + gridDim.x), tile.fixed) | dt);
+            const Amp<T> *ns = a + (cosetTileBase<8 + R>((uint64_t)(w + gridDim.x), tile.fixed) | dt);
 #pragma unroll
             for (int k = 0; k < E; ++k) next[k] = ns[tile.hi[k]];
         }
@@ -409,7 +426,13 @@ __global__ void __launch_bounds__(kBlock) kPauliCoset(Amp<T> *__restrict__ a, un
             const Amp<T> v = st[tx + (k << 8)];
             src[tile.hi[k]] = Amp<T>{sc * v.x, sc * v.y};
         }
-        if (PREFETCH) __syncthreads();  // the next tile overwrites the LDS copy
+        // The prefetched tile goes to LDS here, after this tile's stores are
+        // issued: the wait for its loads then leaves those stores in flight.
+        if (PREFETCH && w + gridDim.x < numTiles) {
+            __syncthreads();  // every wave has read this tile from LDS
+#pragma unroll
+            for (int k = 0; k < E; ++k) st[tx + (k << 8)] = next[k];
+        }
     }
 }
 
