@@ -51,7 +51,7 @@ __device__ __forceinline__ uint64_t fusedGroupBase(uint64_t g, const BitList &t)
     return g;
 }
 
-__global__ void __launch_bounds__(kBlock) kFused5(Amp<float> *__restrict__ a, long long numChunks,
+__global__ void __launch_bounds__(kBlock, 5) kFused5(Amp<float> *__restrict__ a, long long numChunks,
                                                   Fused5Spec spec, const float *__restrict__ M) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
