@@ -3,8 +3,11 @@
 Usage (from the repository root, after the rocprofv3 runs named in the
 summary headers):
     python3 -m quantumlab_amd.profile_summary --name brick32_fp64 \\
-        --trace gpurun_out/prof_trace --fetch gpurun_out/prof_fetch \\
-        --write gpurun_out/prof_write --out profiles
+        --trace <trace dir> --fetch <FETCH_SIZE dir> --write <WRITE_SIZE dir> \\
+        --out profiles
+
+The rocprofv3 commands recorded in a summary write into the directories
+given here (their -d argument).
 
 FETCH_SIZE / WRITE_SIZE (KiB per dispatch) are calibrated against the one
 kernel of known traffic in the run: kScale, the in-place renormalisation
@@ -65,9 +68,9 @@ def main():
         return
     # the state-vector legs only (per-gate FP64 and fused FP32), so the sweep
     # kernels' averages are those of the 32-qubit registers
-    trace_cmd = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_trace -o bench -- "
+    trace_cmd = (f"rocprofv3 --kernel-trace --stats --output-format csv -d {args.trace} -o bench -- "
                  "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-swap --no-density --no-trotter --fp32-qubits 0")
-    pmc_cmd = ("rocprofv3 --pmc {c} --output-format csv -d gpurun_out/prof_{d} -o bench -- "
+    pmc_cmd = ("rocprofv3 --pmc {c} --output-format csv -d {d} -o bench -- "
                "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-fused --no-swap --no-density --no-trotter")
     fetch, calls = counter_means(args.fetch)
     write, _ = counter_means(args.write)
@@ -79,10 +82,10 @@ def main():
     f_fac = known_kib / fetch[cal[0]]
     w_fac = known_kib / write[cal[0]]
     lines = [f"# {args.name}: HBM traffic per dispatch (rocprofv3 PMC)", "",
-             "Commands:", f"- `{pmc_cmd.format(c='FETCH_SIZE', d='fetch')}`",
-             f"- `{pmc_cmd.format(c='WRITE_SIZE', d='write')}`",
-             f"- `python3 -m quantumlab_amd.profile_summary --name {args.name} --trace gpurun_out/prof_trace "
-             f"--fetch gpurun_out/prof_fetch --write gpurun_out/prof_write --out profiles`", "",
+             "Commands:", f"- `{pmc_cmd.format(c='FETCH_SIZE', d=args.fetch)}`",
+             f"- `{pmc_cmd.format(c='WRITE_SIZE', d=args.write)}`",
+             f"- `python3 -m quantumlab_amd.profile_summary --name {args.name} --trace {args.trace} "
+             f"--fetch {args.fetch} --write {args.write} --out {args.out}`", "",
              f"Calibration: kScale<double> (in-place sweep of 2^{args.qubits} FP64 amplitudes, known "
              f"{known_kib / 2**20:.1f} GiB read and {known_kib / 2**20:.1f} GiB written) reads "
              f"FETCH_SIZE {fetch[cal[0]] / 2**20:.3f} GiB and WRITE_SIZE {write[cal[0]] / 2**20:.3f} GiB, so "

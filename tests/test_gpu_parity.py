@@ -299,7 +299,7 @@ def test_fp32_mfma_five_qubit_unitary_matches_fp64(gpu, n):
 
 def test_fused_fp32_mfma_circuit_matches_fp64(gpu):
     """Brick circuit fused into 5-qubit MFMA blocks on FP32 amplitudes against
-    the per-gate FP64 run: fidelity error <= 1e-6 (north-star bar), max-abs
+    the per-gate FP64 run: fidelity error <= 1e-6, max-abs
     amplitude error at FP32 resolution."""
     n = 18
     gates = W.brick_circuit(n, 15, seed=13, extra_every=5)
