@@ -336,22 +336,13 @@ __device__ __forceinline__ void cosetDiagonal(CosetRegs<T, E> &x, T c, T kap, un
     }
 }
 
-// Tile base index: w with zeros inserted at the tile's positions. Coset
-// tiles always hold 8 + R bits, a constant count, so the loop unrolls over
-// scalar positions (a loop over a run-time count re-reads each position from
-// the kernel arguments, and the compiler then waits for every outstanding
-// store before the prefetched tile is written to LDS).
-template <int TB>
-__device__ __forceinline__ uint64_t cosetTileBase(uint64_t w, const BitList &b) {
-#pragma unroll
-    for (int j = 0; j < TB; ++j) w = insertZero(w, b.pos[j]);
-    return w;
-}
-
-// Tile staging, prefetch and the final scaled store as in kPauliBlock; the
-// groups replace its passes.
-template <typename T, bool PREFETCH>
-__global__ void __launch_bounds__(kBlock) kPauliCoset(Amp<T> *__restrict__ a, unsigned numTiles, PauliTile tile,
+// Tile staging and the final scaled store as in kPauliBlock, one tile per
+// workgroup; the groups replace its passes. A persistent FP32 grid that
+// prefetched the next tile into registers measured slower once the groups had
+// cut the LDS round trips (24 qubits: 290 vs 284 ms; 27 qubits: 1470 vs 1298
+// ms; 29 qubits: 6318 vs 5580 ms for the same solves, identical states).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kPauliCoset(Amp<T> *__restrict__ a, PauliTile tile,
                                                       const CosetGroupDev *__restrict__ groups, int numGroups,
                                                       const CosetExpDev *__restrict__ ops, double scale) {
     constexpr int R = cosetBits<T>();
@@ -363,76 +354,50 @@ __global__ void __launch_bounds__(kBlock) kPauliCoset(Amp<T> *__restrict__ a, un
 #pragma unroll
     for (int j = 0; j < 8; ++j)
         if ((tx >> j) & 1) dt |= 1ull << tile.pos[j];
-    Amp<T> next[PREFETCH ? E : 1];
-    unsigned w = blockIdx.x;
-    if (PREFETCH && w < numTiles) {
-        const Amp<T> *ns = a + (cosetTileBase<8 + R>((uint64_t)w, tile.fixed) | dt);
+    const uint64_t base = insertZeros((uint64_t)flatBlockLinear(), tile.fixed);
+    Amp<T> *src = a + (base | dt);
 #pragma unroll
-        for (int k = 0; k < E; ++k) st[tx + (k << 8)] = ns[tile.hi[k]];
-    }
-    for (; w < numTiles; w += PREFETCH ? gridDim.x : numTiles) {
-        // one tile per workgroup without prefetch: the loop form costs one
-        // register less there
-        const uint64_t base = PREFETCH ? cosetTileBase<8 + R>((uint64_t)w, tile.fixed)
-                                       : insertZeros((uint64_t)w, tile.fixed);
-        Amp<T> *src = a + (base | dt);
-        if (!PREFETCH) {
+    for (int k = 0; k < E; ++k) st[tx + (k << 8)] = src[tile.hi[k]];
+    __syncthreads();
+    for (int g = 0; g < numGroups; ++g) {
+        const CosetGroupDev &grp = groups[g];
+        unsigned r = tx;
 #pragma unroll
-            for (int k = 0; k < E; ++k) st[tx + (k << 8)] = src[tile.hi[k]];
+        for (int j = 0; j < R; ++j) {
+            const int p = grp.piv[j];
+            r = ((r >> p) << (p + 1)) | (r & ((1u << p) - 1u));
         }
+        CosetRegs<T, E> x;
+#pragma unroll
+        for (int v = 0; v < E; ++v) {
+            const Amp<T> t = st[r ^ grp.C[v]];
+            x.re[v] = t.x;
+            x.im[v] = t.y;
+        }
+        const int first = grp.first, count = grp.count;
+        for (int o = 0; o < count; ++o) {
+            const CosetExpDev e = ops[first + o];
+            const unsigned p0 = (unsigned)__popcll(base & e.z) + (unsigned)__popc(r & e.lz);
+            const T kap = signFlip((T)e.kap, p0);
+            const T c = (T)e.c;
+            switch (e.kind) {
+            case 0: cosetDiagonal<T, E, false>(x, c, kap, e.zv); break;
+            case 1: cosetExchange<T, E, 1>(x, e.m, c, kap, e.zv); break;
+            case 2: cosetExchange<T, E, 2>(x, e.m, c, kap, e.zv); break;
+            case 3: cosetDiagonal<T, E, true>(x, c, kap, e.zv); break;
+            case 4: cosetExchange<T, E, 4>(x, e.m, c, kap, e.zv); break;
+            default: cosetExchange<T, E, 5>(x, e.m, c, kap, e.zv); break;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < E; ++v) st[r ^ grp.C[v]] = Amp<T>{x.re[v], x.im[v]};
         __syncthreads();
-        if (PREFETCH && w + gridDim.x < numTiles) {
-            const Amp<T> *ns = a + (cosetTileBase<8 + R>((uint64_t)(w + gridDim.x), tile.fixed) | dt);
+    }
+    const T sc = (T)scale;
 #pragma unroll
-            for (int k = 0; k < E; ++k) next[k] = ns[tile.hi[k]];
-        }
-        for (int g = 0; g < numGroups; ++g) {
-            const CosetGroupDev &grp = groups[g];
-            unsigned r = tx;
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int p = grp.piv[j];
-                r = ((r >> p) << (p + 1)) | (r & ((1u << p) - 1u));
-            }
-            CosetRegs<T, E> x;
-#pragma unroll
-            for (int v = 0; v < E; ++v) {
-                const Amp<T> t = st[r ^ grp.C[v]];
-                x.re[v] = t.x;
-                x.im[v] = t.y;
-            }
-            const int first = grp.first, count = grp.count;
-            for (int o = 0; o < count; ++o) {
-                const CosetExpDev e = ops[first + o];
-                const unsigned p0 = (unsigned)__popcll(base & e.z) + (unsigned)__popc(r & e.lz);
-                const T kap = signFlip((T)e.kap, p0);
-                const T c = (T)e.c;
-                switch (e.kind) {
-                case 0: cosetDiagonal<T, E, false>(x, c, kap, e.zv); break;
-                case 1: cosetExchange<T, E, 1>(x, e.m, c, kap, e.zv); break;
-                case 2: cosetExchange<T, E, 2>(x, e.m, c, kap, e.zv); break;
-                case 3: cosetDiagonal<T, E, true>(x, c, kap, e.zv); break;
-                case 4: cosetExchange<T, E, 4>(x, e.m, c, kap, e.zv); break;
-                default: cosetExchange<T, E, 5>(x, e.m, c, kap, e.zv); break;
-                }
-            }
-#pragma unroll
-            for (int v = 0; v < E; ++v) st[r ^ grp.C[v]] = Amp<T>{x.re[v], x.im[v]};
-            __syncthreads();
-        }
-        const T sc = (T)scale;
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const Amp<T> v = st[tx + (k << 8)];
-            src[tile.hi[k]] = Amp<T>{sc * v.x, sc * v.y};
-        }
-        // The prefetched tile goes to LDS here, after this tile's stores are
-        // issued: the wait for its loads then leaves those stores in flight.
-        if (PREFETCH && w + gridDim.x < numTiles) {
-            __syncthreads();  // every wave has read this tile from LDS
-#pragma unroll
-            for (int k = 0; k < E; ++k) st[tx + (k << 8)] = next[k];
-        }
+    for (int k = 0; k < E; ++k) {
+        const Amp<T> v = st[tx + (k << 8)];
+        src[tile.hi[k]] = Amp<T>{sc * v.x, sc * v.y};
     }
 }
 
@@ -735,13 +700,13 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seqIn, int countIn, int rep
             const size_t lds = (q->prec == FP64 ? sizeof(Amp<double>) : sizeof(Amp<float>)) << b.tile.tb;
             if (b.coset) {
                 if (q->prec == FP64)
-                    hipLaunchKernelGGL((kPauliCoset<double, false>), dim3(tiles), dim3(kBlock), lds, hstream(),
-                                       (Amp<double> *)q->amps, tiles, b.tile, dGroups + b.firstPass, b.numPasses,
+                    hipLaunchKernelGGL(kPauliCoset<double>, dim3(tiles), dim3(kBlock), lds, hstream(),
+                                       (Amp<double> *)q->amps, b.tile, dGroups + b.firstPass, b.numPasses,
                                        dCops + b.firstOp, b.scale);
                 else
-                    hipLaunchKernelGGL((kPauliCoset<float, true>), dim3(std::min(tiles, prefetchGrid)), dim3(kBlock),
-                                       lds, hstream(), (Amp<float> *)q->amps, tiles, b.tile, dGroups + b.firstPass,
-                                       b.numPasses, dCops + b.firstOp, b.scale);
+                    hipLaunchKernelGGL(kPauliCoset<float>, dim3(tiles), dim3(kBlock), lds, hstream(),
+                                       (Amp<float> *)q->amps, b.tile, dGroups + b.firstPass, b.numPasses,
+                                       dCops + b.firstOp, b.scale);
                 continue;
             }
             // pairs per thread: 2^tb / 2 / kBlock at the full tile size (8 FP32, 4 FP64)
