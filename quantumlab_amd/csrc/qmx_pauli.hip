@@ -72,8 +72,10 @@ __device__ __forceinline__ double signFlip(double v, unsigned p) {
 // persistent grid (4 workgroups per CU, the occupancy its 119 VGPRs allow)
 // walks tiles w = blockIdx.x + k gridDim.x and loads the next tile into
 // registers while the current one is in LDS: 14 % faster for FP32 at 24
-// qubits. FP64 runs one workgroup per tile (its prefetch buffer was placed in
-// scratch memory, and the variant without it is faster).
+// qubits, measured while this form served full tiles. FP64 runs one workgroup
+// per tile (its prefetch buffer was placed in scratch memory, and the variant
+// without it is faster). Full tiles now take the coset form, so this kernel
+// serves registers smaller than a tile, which are a single tile.
 template <typename T, int PPT, bool PREFETCH>
 __global__ void __launch_bounds__(kBlock) kPauliBlock(Amp<T> *__restrict__ a, unsigned numTiles, PauliTile tile,
                                                       const PassDev *__restrict__ passes, int numPasses,
