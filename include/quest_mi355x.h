@@ -76,12 +76,66 @@ int qmxSampleShots(Qureg qureg, const int *qubits, int numQubits, int shots, int
 qreal qmxMaxAbsDiff(Qureg a, Qureg b);
 /* FP64 norm of the register; if rescale != 0 the register is renormalised */
 qreal qmxRenormalise(Qureg qureg, int rescale);
+/* QuantumLab's wave-function CSV (FermionSimulation/Fermion1D.cpp:13-83,
+ * SaveWaveFunction / LoadWaveFunction): one "%f, %f" line (real, imaginary) per
+ * amplitude, no header. Save writes the register's amplitudes (density
+ * registers: the flat 4^n entries); returns 1. Load reads such a file into the
+ * register; the line count must equal its amplitude count; returns the
+ * amplitudes read, or 0 with the error reported. */
+int qmxSaveStateCSV(Qureg qureg, const char *path);
+long long qmxLoadStateCSV(Qureg qureg, const char *path);
 
-/* ---- circuits and fusion ------------------------------------------------------ */
+/* ---- circuits and fusion ------------------------------------------------------
+ * A circuit is an operation list for registers of one width, recorded once
+ * (and validated then, with QuEST's messages) and applied to any register of
+ * that width. It replaces QuantumLab's loop over QLGate::PerformBasicOperation
+ * (QL/Circuit/QLGate.cpp:652-803, driven by QL/Simulator/QLSimulator*.cpp):
+ * unfused, each operation is the QuEST call that loop makes; fused (state
+ * vectors), the unitaries between collapses run as dense blocks of at most k
+ * qubits, one sweep per block. */
 typedef struct qmxCircuit qmxCircuit;
+
+/* QuantumLab's basic operations: the codes of EBasicOperation
+ * (QL/Circuit/QLGate.h:18-104). Two-qubit gates list the control first.
+ * param: the angle (P, RX, RY, RZ, CP, CRX, CRY, CRZ), the outcome (CC) or the
+ * probability (noise). Noise acts on density registers only and is ignored on
+ * state vectors; PHASE and COMPOSITE are accepted and not dispatched, as in
+ * PerformBasicOperation. */
+enum qmxOperationCode {
+    QMX_OP_H = 0, QMX_OP_X = 1, QMX_OP_Y = 2, QMX_OP_Z = 3, QMX_OP_P = 4, QMX_OP_PHASE = 5,
+    QMX_OP_RX = 6, QMX_OP_RY = 7, QMX_OP_RZ = 8,
+    QMX_OP_CX = 9, QMX_OP_CY = 10, QMX_OP_CZ = 11, QMX_OP_CP = 12, QMX_OP_CRX = 13, QMX_OP_CRY = 14,
+    QMX_OP_CRZ = 15, QMX_OP_CC = 16,
+    QMX_OP_NOISE_DAMPING = 17, QMX_OP_NOISE_DEPHASING = 18, QMX_OP_NOISE_DEPOLARISING = 19,
+    QMX_OP_NOISE_MIX_PAULI_X = 20, QMX_OP_NOISE_MIX_PAULI_Y = 21, QMX_OP_NOISE_MIX_PAULI_Z = 22,
+    QMX_OP_NOISE_MIX_PAULI_ALL = 23, QMX_OP_NOISE_TWO_QUBIT_DEPHASING = 24,
+    QMX_OP_NOISE_TWO_QUBIT_DEPOLARISING = 25, QMX_OP_COMPOSITE = 26
+};
+
+/* one SBasicOperation (QL/Circuit/QLGate.h:106-112): m_eOperation,
+ * m_lstQubits (at most two), m_fClassicalParameter */
+typedef struct qmxBasicOperation {
+    int operation;
+    int numQubits;
+    int qubits[2];
+    qreal param;
+} qmxBasicOperation;
+
+/* The density simulator's noise after every operation that is neither noise
+ * nor a collapse (QLSimulatorDensityMatrix.cpp:88-153; the members of its
+ * parameters m_fDampingAfterGate .. m_fTwoDepolarisingAfterGate). Applied to
+ * density registers only; zero disables a channel. */
+typedef struct qmxNoiseAfterGate {
+    qreal damping, dephasing, depolarising, mixPauli, twoQubitDephasing, twoQubitDepolarising;
+} qmxNoiseAfterGate;
+
+/* NULL (with the error reported) for an invalid width */
 qmxCircuit *qmxCircuitCreate(int numQubits);
 void qmxCircuitDestroy(qmxCircuit *c);
+/* operations recorded, of every kind */
 int qmxCircuitNumGates(const qmxCircuit *c);
+void qmxCircuitAddOperation(qmxCircuit *c, int operation, const int *qubits, int numQubits, qreal param);
+void qmxCircuitSetNoiseAfterGate(qmxCircuit *c, qmxNoiseAfterGate noise);
 void qmxCircuitCompactUnitary(qmxCircuit *c, int target, Complex alpha, Complex beta);
 void qmxCircuitUnitary(qmxCircuit *c, int target, ComplexMatrix2 u);
 void qmxCircuitRotateZ(qmxCircuit *c, int target, qreal angle);
@@ -89,13 +143,24 @@ void qmxCircuitControlledNot(qmxCircuit *c, int control, int target);
 void qmxCircuitMultiControlledUnitary(qmxCircuit *c, int *controls, int numControls, int target, ComplexMatrix2 u);
 /* u is row-major 2^k x 2^k, re[r*dim+c]; targs[0] = least significant bit */
 void qmxCircuitMultiQubitUnitary(qmxCircuit *c, int *targs, int numTargs, const qreal *re, const qreal *im);
-/* group the gate list into dense blocks of at most maxQubits qubits;
- * returns the number of blocks */
+/* group the unitaries into dense blocks of at most maxQubits (1..10) qubits,
+ * collapses acting as barriers; returns the number of steps (blocks plus
+ * collapses) */
 int qmxCircuitFuse(qmxCircuit *c, int maxQubits);
+/* dense blocks made by the last qmxCircuitFuse */
 int qmxCircuitNumBlocks(const qmxCircuit *c);
-/* apply to a state vector: fused != 0 uses the fused blocks (qmxCircuitFuse
- * must have been called), otherwise one QuEST-equivalent kernel per gate */
-void qmxCircuitApply(Qureg qureg, qmxCircuit *c, int fused);
+/* Apply to a register of the circuit's width; returns the product of the
+ * collapse probabilities (1 without collapses). fused != 0 on a state vector
+ * uses the fused blocks (after qmxCircuitFuse), otherwise each operation is the
+ * QuEST call PerformBasicOperation makes; density registers always run the
+ * operations in order, each followed by the circuit's after-gate noise. */
+qreal qmxCircuitApply(Qureg qureg, qmxCircuit *c, int fused);
+/* The simulators' loop over an SBasicOperation list in one call: records ops
+ * (validated), fuses into blocks of at most fuseQubits qubits when that is
+ * non-zero and the register is a state vector, applies, and returns the
+ * product of the collapse probabilities. noise may be NULL. */
+qreal qmxApplyOperations(Qureg qureg, const qmxBasicOperation *ops, int numOps, const qmxNoiseAfterGate *noise,
+                         int fuseQubits);
 
 /* ---- batched swap tests (quantum k-means) ------------------------------------ */
 /* A plan holds numPoints real vectors (point p has 2^pointQubits[p] entries,

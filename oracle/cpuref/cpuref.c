@@ -437,6 +437,16 @@ void pauliY(Qureg q, int t) {
     sv_matrix2(q, t, 0, 0, ur, ui);
     if (q.isDensityMatrix) sv_matrix2(q, t + shiftOf(q), 0, 0, ur, uc);
 }
+/* QuEST.c:569-578, QuEST_cpu.c:2965-3016: {{0,-i},{i,0}} on t where bit c is 1;
+ * density registers: its conjugate on the column qubits (conjFac = -1) */
+void controlledPauliY(Qureg q, int c, int t) {
+    qreal ur[2][2] = {{0, 0}, {0, 0}}, ui[2][2] = {{0, -1}, {1, 0}}, uc[2][2] = {{0, 1}, {-1, 0}};
+    sv_matrix2(q, t, 1LL << c, 0, ur, ui);
+    if (q.isDensityMatrix) {
+        int s = shiftOf(q);
+        sv_matrix2(q, t + s, 1LL << (c + s), 0, ur, uc);
+    }
+}
 static void phaseBoth(Qureg q, ll mask, qreal cr, qreal ci) {
     sv_phase(q, mask, cr, ci);
     if (q.isDensityMatrix) sv_phase(q, mask << shiftOf(q), cr, -ci);

@@ -138,6 +138,10 @@ double densInnerProduct(const QuregDev *a, const QuregDev *b);     // Σ Re(a* b
 double hsDistanceSquared(const QuregDev *a, const QuregDev *b);
 double densFidelity(const QuregDev *rho, const QuregDev *psi);     // <psi|rho|psi>
 void probOfAllOutcomes(const QuregDev *q, const int *qubits, int k, double *out);
+// device doubles probOfAllOutcomesInto needs as scratch for these qubits
+long long outcomeScratchDoubles(const QuregDev *q, const int *qubits, int k);
+// the same histogram with a caller-provided device scratch (outcomeScratchDoubles)
+void probOfAllOutcomesInto(const QuregDev *q, const int *qubits, int k, double *scratch, double *out);
 double normSquaredThenScale(QuregDev *q, bool rescale);            // FP64 norm, optional 1/sqrt renorm
 double maxAbsDiff(const QuregDev *a, const QuregDev *b);           // max_i |a_i - b_i|
 // <D> of a diagonal operator given as device FP64 arrays of 2^numQubitsRep entries

@@ -550,6 +550,7 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seqIn, int countIn, int rep
     const int count = (int)packed.size();
     const int R = q->prec == FP64 ? cosetBits<double>() : cosetBits<float>();
     const bool coset = tb == 8 + R;                           // full tiles: the coset form
+    const double scaleFloor = q->prec == FP64 ? std::ldexp(1.0, -500) : std::ldexp(1.0, -40);
     std::vector<PauliExpDev> ops;
     std::vector<PassDev> passes;
     std::vector<CosetExpDev> cops;
@@ -619,8 +620,12 @@ void applyPauliSequence(QuregDev *q, const PauliExp *seqIn, int countIn, int rep
                 d.kind = 2;
                 d.kap = (e.ny & 2) ? sn : -sn;
             }
-            // |tan(theta/2)| <= sqrt(3): divide by cos, scale the tile by it once
-            if (std::fabs(d.c) >= 0.5) {
+            // |tan(theta/2)| <= sqrt(3): divide by cos, scale the tile by it once.
+            // Amplitudes grow by up to 1/|cos| per such rotation until the final
+            // scaled store, so the running product is kept above a floor (amplitudes
+            // stay below 2^40 in FP32, 2^500 in FP64, and the scale a normal number);
+            // rotations past it use the plain form.
+            if (std::fabs(d.c) >= 0.5 && std::fabs(blk.scale * d.c) >= scaleFloor) {
                 d.kind += 3;
                 d.kap /= d.c;
                 blk.scale *= d.c;

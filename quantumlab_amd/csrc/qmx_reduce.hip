@@ -168,35 +168,72 @@ __global__ void __launch_bounds__(kBlock) kFidelity(const Amp<TR> *__restrict__ 
     writePartial<1>(acc, part);
 }
 
-// histogram of probabilities over the outcome bits (statevec or density diag)
-// QuEST_cpu.c calcProbOfAllOutcomesLocal; one pass per 64 outcomes
+// Outcome histogram in one pass over the register (QuEST_cpu.c
+// calcProbOfAllOutcomesLocal makes one pass too). Index i = (hi << 6) | lane:
+// a lane keeps the index bits below 6 fixed, so the outcome bits among them
+// (lowMask) are fixed per lane, and a wave walks the hi values that carry one
+// combination g of the outcome bits at or above 6 (highPos, in hi space),
+// split into 2^chunkLog chunks of 2^lenLog values. Each lane sums its
+// probabilities with Neumaier compensation; lanes sharing their low outcome
+// bits merge by a butterfly over the other lane bits, and the lane whose other
+// bits are zero writes the chunk's partial of that outcome. Loads are
+// contiguous 64-amplitude runs for any choice of qubits. Deterministic.
+struct OutcomePlan {
+    BitList highPos;        // outcome qubits >= 6, minus 6, ascending
+    int qpos[kMaxBits];     // qubits[j] in the caller's order: outcome bit j
+    int k;
+    unsigned lowMask;       // lane bits that are outcome qubits
+    int chunkLog, lenLog;
+    long long units;        // 2^(high outcome bits) x 2^chunkLog
+};
+
 template <typename T, bool DENS>
-__global__ void __launch_bounds__(kBlock) kAllOutcomes(const Amp<T> *__restrict__ a, long long work,
-                                                       long long dim, BitList qubits, int first,
-                                                       int count, double *part) {
-    Acc h[1];
-    for (int o = 0; o < count; ++o) {
-        h[0] = Acc();
-        const int outcome = first + o;
-        for (long long w = (long long)blockIdx.x * kBlock + threadIdx.x; w < work;
-             w += (long long)gridDim.x * kBlock) {
-            uint64_t idx = (uint64_t)w;
-            int got = 0;
-            for (int j = 0; j < qubits.n; ++j) got |= (int)((idx >> qubits.pos[j]) & 1ull) << j;
-            if (got != outcome) continue;
+__global__ void __launch_bounds__(kBlock) kOutcomeHist(const Amp<T> *__restrict__ a, long long work, long long dim,
+                                                       OutcomePlan pl, double *part) {
+    const unsigned lane = threadIdx.x & 63u;
+    const long long numWaves = ((long long)gridDim.x * kBlock) >> 6;
+    for (long long u = (((long long)blockIdx.x * kBlock) + threadIdx.x) >> 6; u < pl.units; u += numWaves) {
+        const uint64_t g = (uint64_t)u >> pl.chunkLog;
+        const uint64_t ch = (uint64_t)u & ((1ull << pl.chunkLog) - 1ull);
+        uint64_t gbits = 0;  // g deposited at the high outcome positions
+        for (int j = 0; j < pl.highPos.n; ++j) gbits |= ((g >> j) & 1ull) << pl.highPos.pos[j];
+        Acc acc[1];
+        const uint64_t len = 1ull << pl.lenLog;
+        const uint64_t first = ((insertZeros(ch << pl.lenLog, pl.highPos) | gbits) << 6) | lane;
+        for (uint64_t t = 0; t < len; ++t) {
+            const uint64_t i = ((insertZeros((ch << pl.lenLog) | t, pl.highPos) | gbits) << 6) | lane;
+            if ((long long)i >= work) continue;
             if (DENS) {
-                h[0].add((double)a[idx * (dim + 1)].x);
+                acc[0].add((double)a[i * (uint64_t)(dim + 1)].x);
             } else {
-                Amp<T> v = a[idx];
-                h[0].add((double)v.x * v.x);
-                h[0].add((double)v.y * v.y);
+                const Amp<T> v = a[i];
+                acc[0].add((double)v.x * v.x);
+                acc[0].add((double)v.y * v.y);
             }
         }
-        double out[1];
-        blockReduce<1>(h, out);
-        if (threadIdx.x == 0) part[(size_t)o * kMaxPartials + blockIdx.x] = out[0];
-        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            if ((pl.lowMask >> b) & 1u) continue;
+            const double os = __shfl_xor(acc[0].s, 1 << b, 64);
+            const double oc = __shfl_xor(acc[0].c, 1 << b, 64);
+            acc[0].merge(os, oc);
+        }
+        if ((lane & ~pl.lowMask) == 0u && (long long)first < work) {
+            uint64_t o = 0;
+            for (int j = 0; j < pl.k; ++j) o |= ((first >> pl.qpos[j]) & 1ull) << j;
+            part[(o << pl.chunkLog) | ch] = acc[0].value();
+        }
     }
+}
+
+// per outcome, the chunks' partials in chunk order
+__global__ void __launch_bounds__(kBlock) kOutcomeFinish(const double *__restrict__ part, int chunkLog,
+                                                         long long numOut, double *out) {
+    const long long o = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (o >= numOut) return;
+    Acc acc;
+    for (long long c = 0; c < (1ll << chunkLog); ++c) acc.add(part[(o << chunkLog) + c]);
+    out[o] = acc.value();
 }
 
 template <int NV>
@@ -328,37 +365,81 @@ double densFidelity(const QuregDev *rho, const QuregDev *psi) {
     return r;
 }
 
-void probOfAllOutcomes(const QuregDev *q, const int *qubits, int k, double *out) {
-    ensureBuffers();
+namespace {
+
+// waves the histogram aims for (16 per CU on 256 CUs), and the grid cap
+constexpr int kOutcomeUnitsLog = 12;
+constexpr long long kOutcomeMaxBlocks = 8192;
+
+OutcomePlan outcomePlan(const QuregDev *q, const int *qubits, int k) {
+    OutcomePlan pl{};
+    const long long work = q->isDensity ? (1ll << q->numQubitsRep) : q->numAmps;
+    const int workBits = 63 - __builtin_clzll((unsigned long long)work);
+    pl.k = k;
+    int high[kMaxBits], nh = 0;
+    for (int j = 0; j < k; ++j) {
+        pl.qpos[j] = qubits[j];
+        if (qubits[j] < 6) pl.lowMask |= 1u << qubits[j];
+        else high[nh++] = qubits[j] - 6;
+    }
+    std::sort(high, high + nh);
+    pl.highPos.n = nh;
+    for (int j = 0; j < nh; ++j) pl.highPos.pos[j] = high[j];
+    const int hiBits = std::max(0, workBits - 6 - nh);
+    pl.chunkLog = std::min(hiBits, std::max(0, kOutcomeUnitsLog - nh));
+    pl.lenLog = hiBits - pl.chunkLog;
+    pl.units = 1ll << (nh + pl.chunkLog);
+    return pl;
+}
+
+}  // namespace
+
+long long outcomeScratchDoubles(const QuregDev *q, const int *qubits, int k) {
+    const OutcomePlan pl = outcomePlan(q, qubits, k);
+    const long long numOut = 1ll << k;
+    return pl.chunkLog ? (numOut << pl.chunkLog) + numOut : numOut;
+}
+
+void probOfAllOutcomesInto(const QuregDev *q, const int *qubits, int k, double *scratch, double *out) {
+    const OutcomePlan pl = outcomePlan(q, qubits, k);
     const bool dens = q->isDensity != 0;
     const long long dim = 1ll << q->numQubitsRep;
     const long long work = dens ? dim : q->numAmps;
-    const unsigned grid = std::min<unsigned>(redGrid(work), 256);
-    BitList qb;
-    qb.n = k;
-    for (int j = 0; j < k; ++j) qb.pos[j] = qubits[j];
-    const int numOut = 1 << k;
-    for (int first = 0; first < numOut; first += 4) {
-        int count = std::min(4, numOut - first);
-        QMX_DISPATCH(q->prec, T, {
-            if (dens)
-                hipLaunchKernelGGL((kAllOutcomes<T, true>), dim3(grid), dim3(kBlock), 0, hstream(),
-                                   (const Amp<T> *)q->amps, work, dim, qb, first, count, g_partials);
-            else
-                hipLaunchKernelGGL((kAllOutcomes<T, false>), dim3(grid), dim3(kBlock), 0, hstream(),
-                                   (const Amp<T> *)q->amps, work, dim, qb, first, count, g_partials);
-        });
-        checkLaunch("probOfAllOutcomes");
-        for (int o = 0; o < count; ++o) {
-            hipLaunchKernelGGL(kFinish<1>, dim3(1), dim3(kBlock), 0, hstream(),
-                               g_partials + (size_t)o * kMaxPartials, (int)grid, g_result + o);
-            checkLaunch("probOfAllOutcomes finish");
-        }
-        double *h = pinnedScratch();
-        QMX_HIP_CHECK(hipMemcpyAsync(h, g_result, count * sizeof(double), hipMemcpyDeviceToHost, hstream()));
-        QMX_HIP_CHECK(hipStreamSynchronize(hstream()));
-        for (int o = 0; o < count; ++o) out[first + o] = h[o];
+    const long long numOut = 1ll << k;
+    double *res = scratch;                            // numOut results
+    double *part = pl.chunkLog ? scratch + numOut : scratch;
+    const long long blocks = std::min(kOutcomeMaxBlocks, (pl.units + 3) / 4);
+    QMX_DISPATCH(q->prec, T, {
+        if (dens)
+            hipLaunchKernelGGL((kOutcomeHist<T, true>), dim3((unsigned)blocks), dim3(kBlock), 0, hstream(),
+                               (const Amp<T> *)q->amps, work, dim, pl, part);
+        else
+            hipLaunchKernelGGL((kOutcomeHist<T, false>), dim3((unsigned)blocks), dim3(kBlock), 0, hstream(),
+                               (const Amp<T> *)q->amps, work, dim, pl, part);
+    });
+    checkLaunch("probOfAllOutcomes");
+    if (pl.chunkLog) {
+        hipLaunchKernelGGL(kOutcomeFinish, dim3((unsigned)((numOut + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           hstream(), part, pl.chunkLog, numOut, res);
+        checkLaunch("probOfAllOutcomes finish");
     }
+    QMX_HIP_CHECK(hipMemcpyAsync(out, res, numOut * sizeof(double), hipMemcpyDeviceToHost, hstream()));
+    QMX_HIP_CHECK(hipStreamSynchronize(hstream()));
+}
+
+void probOfAllOutcomes(const QuregDev *q, const int *qubits, int k, double *out) {
+    static double *buf = nullptr;
+    static long long bufDoubles = 0;
+    const long long need = outcomeScratchDoubles(q, qubits, k);
+    if (need > bufDoubles) {
+        if (buf) {
+            deviceSync();
+            devFree(buf);
+        }
+        buf = static_cast<double *>(devAlloc((size_t)need * sizeof(double)));
+        bufDoubles = need;
+    }
+    probOfAllOutcomesInto(q, qubits, k, buf, out);
 }
 
 double normSquaredThenScale(QuregDev *q, bool rescale) {

@@ -10,10 +10,12 @@
 #include <sys/time.h>
 #include <unistd.h>
 
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "QuEST.h"
@@ -540,11 +542,15 @@ Qureg createDensityQureg(int numQubits, QuESTEnv env) {
 }
 
 Qureg createCloneQureg(Qureg qureg, QuESTEnv env) {
+    Qureg empty;
+    std::memset(&empty, 0, sizeof empty);
+    API_TRY
     (void)env;
     Qureg q = makeQureg(qureg.numQubitsInStateVec, qureg.isDensityMatrix, qureg.numQubitsRepresented, D(qureg)->prec,
                         __func__);
     qmx::copyState(D(q), D(qureg));
     return q;
+    API_CATCH(empty)
 }
 
 void destroyQureg(Qureg qureg, QuESTEnv env) {
@@ -612,60 +618,109 @@ void destroyPauliHamil(PauliHamil h) {
     std::free(h.pauliCodes);
 }
 
-// file format (QuEST.c:1459-1541): one term per line, "coeff code code ..."
+// ---- PauliHamil text files -------------------------------------------------------
+//
+// The format read by QuEST's createPauliHamilFromFile (QuEST.c:1459-1541): one
+// term per line, "coeff code_0 code_1 ... code_{n-1}", codes 0-3 for I, X, Y, Z.
+// The rules below give the same register width, term count, values and
+// errors as that reader:
+//  - width: the fields of the first line, split on ' ' (tabs and '\r' belong to
+//    a field), minus the coefficient; a line that begins with ' ' counts one
+//    field more (an empty coefficient field before that space);
+//  - terms: the non-empty lines ('\n'-separated; a line of blanks counts);
+//  - values: the whole file is one stream of numbers separated by any white
+//    space, read in order (coefficient, then n codes, per term), each number
+//    in C's strtod / strtol syntax; the first that does not parse is an error.
+namespace {
+
+struct HamilText {
+    std::string text;
+    size_t at = 0;
+    void skipSpace() {
+        while (at < text.size() && std::isspace((unsigned char)text[at])) ++at;
+    }
+    bool real(qreal *out) {
+        skipSpace();
+        const char *begin = text.c_str() + at;
+        char *end = nullptr;
+        const qreal v = sizeof(qreal) == sizeof(float) ? (qreal)std::strtof(begin, &end) : (qreal)std::strtod(begin, &end);
+        if (end == begin) return false;
+        at += (size_t)(end - begin);
+        *out = v;
+        return true;
+    }
+    bool integer(int *out) {
+        skipSpace();
+        const char *begin = text.c_str() + at;
+        char *end = nullptr;
+        const long v = std::strtol(begin, &end, 10);
+        if (end == begin) return false;
+        at += (size_t)(end - begin);
+        *out = (int)v;
+        return true;
+    }
+};
+
+int hamilFileWidth(const std::string &text) {
+    const std::string first = text.substr(0, text.find('\n'));
+    int fields = 0;
+    bool inField = false;
+    for (char ch : first) {
+        const bool blank = ch == ' ';
+        if (!blank && !inField) ++fields;
+        inField = !blank;
+    }
+    if (!first.empty() && first[0] == ' ') ++fields;
+    return fields - 1;
+}
+
+int hamilFileTerms(const std::string &text) {
+    int terms = 0;
+    size_t lineStart = 0;
+    while (lineStart <= text.size()) {
+        size_t nl = text.find('\n', lineStart);
+        if (nl == std::string::npos) nl = text.size();
+        terms += nl > lineStart;
+        lineStart = nl + 1;
+    }
+    return terms;
+}
+
+}  // namespace
+
 PauliHamil createPauliHamilFromFile(char *fn) {
     PauliHamil h;
     std::memset(&h, 0, sizeof h);
     API_TRY
-    FILE *file = std::fopen(fn, "r");
-    qv::fileOpened(file != nullptr, fn, __func__);
-    int numQubits = -1;
-    int ch = std::getc(file);
-    char prev = '0';
-    while (ch != '\n' && ch != EOF) {
-        if (ch == ' ' && prev != ' ') numQubits++;
-        prev = (char)ch;
-        ch = std::getc(file);
+    HamilText in;
+    {
+        FILE *file = std::fopen(fn, "rb");
+        qv::fileOpened(file != nullptr, fn, __func__);
+        char buf[65536];
+        size_t got;
+        while ((got = std::fread(buf, 1, sizeof buf, file)) > 0) in.text.append(buf, got);
+        std::fclose(file);
     }
-    if (prev != ' ') numQubits++;
-    int numTerms = 0;
-    prev = '\n';
-    std::rewind(file);
-    while ((ch = std::getc(file)) != EOF) {
-        if (ch == '\n' && prev != '\n') numTerms++;
-        prev = (char)ch;
-    }
-    if (prev != '\n') numTerms++;
-    if (!(numQubits > 0 && numTerms > 0)) std::fclose(file);
+    const int numQubits = hamilFileWidth(in.text);
+    const int numTerms = hamilFileTerms(in.text);
     qv::checkFmt(numQubits > 0 && numTerms > 0, qv::E_INVALID_PAULI_HAMIL_FILE_PARAMS, __func__, fn);
-    h = createPauliHamil(numQubits, numTerms);
-    std::rewind(file);
-    for (int t = 0; t < numTerms; ++t) {
-        bool ok = std::fscanf(file, "%lf ", &h.termCoeffs[t]) == 1;
-        if (!ok) {
-            destroyPauliHamil(h);
-            std::fclose(file);
-        }
-        qv::checkFmt(ok, qv::E_CANNOT_PARSE_PAULI_HAMIL_FILE_COEFF, __func__, fn);
-        for (int q = 0; q < numQubits; ++q) {
-            int code = 0;
-            ok = std::fscanf(file, "%d ", &code) == 1;
-            h.pauliCodes[t * numQubits + q] = (enum pauliOpType)code;
-            if (!ok) {
-                destroyPauliHamil(h);
-                std::fclose(file);
+    PauliHamil parsed = createPauliHamil(numQubits, numTerms);
+    try {
+        for (int t = 0; t < numTerms; ++t) {
+            qv::checkFmt(in.real(&parsed.termCoeffs[t]), qv::E_CANNOT_PARSE_PAULI_HAMIL_FILE_COEFF, __func__, fn);
+            for (int q = 0; q < numQubits; ++q) {
+                int code = 0;
+                qv::checkFmt(in.integer(&code), qv::E_CANNOT_PARSE_PAULI_HAMIL_FILE_PAULI, __func__, fn);
+                qv::checkFmtCode(qv::validPauliCode(code), qv::E_INVALID_PAULI_HAMIL_FILE_PAULI_CODE, __func__, fn,
+                                 code);
+                parsed.pauliCodes[t * numQubits + q] = (enum pauliOpType)code;
             }
-            qv::checkFmt(ok, qv::E_CANNOT_PARSE_PAULI_HAMIL_FILE_PAULI, __func__, fn);
-            bool valid = qv::validPauliCode(code);
-            if (!valid) {
-                destroyPauliHamil(h);
-                std::fclose(file);
-            }
-            qv::checkFmtCode(valid, qv::E_INVALID_PAULI_HAMIL_FILE_PAULI_CODE, __func__, fn, code);
         }
+    } catch (const qmx::InputError &) {
+        destroyPauliHamil(parsed);
+        throw;
     }
-    std::fclose(file);
-    return h;
+    return parsed;
     API_CATCH(h)
 }
 
@@ -2126,5 +2181,77 @@ void qmxTimerBegin(void) { qmx::timerBegin(); }
 double qmxTimerEndMs(void) { return qmx::timerEndMs(); }
 
 qreal qmxRenormalise(Qureg qureg, int rescale) { return qmx::normSquaredThenScale(D(qureg), rescale != 0); }
+
+// ---- wave-function CSV files ------------------------------------------------------
+// QuantumLab's format (FermionSimulation/Fermion1D.cpp:13-83): one amplitude per
+// line, "%f, %f" (real, imaginary), no header. A line is read as two fields
+// split on ',', blanks and tabs around them ignored; anything else is an error.
+
+int qmxSaveStateCSV(Qureg qureg, const char *path) {
+    API_TRY
+    FILE *f = std::fopen(path, "w");
+    qv::fileOpened(f != nullptr, path, __func__);
+    const long long n = qureg.numAmpsTotal;
+    constexpr long long kChunk = 1 << 20;
+    std::vector<double> re((size_t)std::min(n, kChunk)), im(re.size());
+    for (long long s = 0; s < n; s += kChunk) {
+        const long long c = std::min(kChunk, n - s);
+        qmx::downloadAmps(D(qureg), s, re.data(), im.data(), c);
+        for (long long i = 0; i < c; ++i) std::fprintf(f, "%f, %f\n", (qreal)re[i], (qreal)im[i]);
+    }
+    std::fclose(f);
+    return 1;
+    API_CATCH(0)
+}
+
+namespace {
+
+// one "re, im" line; false if it is not exactly two numeric fields
+bool csvAmp(const std::string &line, double *re, double *im) {
+    const size_t comma = line.find(',');
+    if (comma == std::string::npos || line.find(',', comma + 1) != std::string::npos) return false;
+    auto field = [](std::string t, double *out) {
+        const size_t a = t.find_first_not_of(" \t\r");
+        if (a == std::string::npos) return false;
+        const size_t b = t.find_last_not_of(" \t\r");
+        t = t.substr(a, b - a + 1);
+        char *end = nullptr;
+        *out = std::strtod(t.c_str(), &end);
+        return end == t.c_str() + t.size();
+    };
+    return field(line.substr(0, comma), re) && field(line.substr(comma + 1), im);
+}
+
+}  // namespace
+
+long long qmxLoadStateCSV(Qureg qureg, const char *path) {
+    API_TRY
+    FILE *f = std::fopen(path, "r");
+    qv::fileOpened(f != nullptr, path, __func__);
+    std::vector<double> re, im;
+    std::string line;
+    int ch;
+    bool ok = true;
+    auto takeLine = [&]() {
+        double r, i;
+        if (!csvAmp(line, &r, &i)) ok = false;
+        re.push_back(r);
+        im.push_back(i);
+        line.clear();
+    };
+    while ((ch = std::fgetc(f)) != EOF) {
+        if (ch == '\n') takeLine();
+        else line.push_back((char)ch);
+        if (!ok) break;
+    }
+    if (ok && !line.empty()) takeLine();
+    std::fclose(f);
+    if (!ok) qv::raise("Invalid wave-function file. Each line must be two numbers separated by a comma.", __func__);
+    if ((long long)re.size() != qureg.numAmpsTotal)
+        qv::raise("The wave-function file's number of amplitudes does not match the register.", __func__);
+    qmx::uploadAmps(D(qureg), 0, re.data(), im.data(), (long long)re.size());
+    return (long long)re.size();
+    API_CATCH(0)
+}
 
 }  // extern "C"

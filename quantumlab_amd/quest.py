@@ -86,6 +86,17 @@ class PauliHamil(C.Structure):
                 ("numSumTerms", C.c_int), ("numQubits", C.c_int)]
 
 
+class BasicOperation(C.Structure):
+    """qmxBasicOperation (include/quest_mi355x.h): one QuantumLab SBasicOperation."""
+    _fields_ = [("operation", C.c_int), ("numQubits", C.c_int), ("qubits", C.c_int * 2), ("param", qreal)]
+
+
+class NoiseAfterGate(C.Structure):
+    """qmxNoiseAfterGate (include/quest_mi355x.h)."""
+    _fields_ = [("damping", qreal), ("dephasing", qreal), ("depolarising", qreal), ("mixPauli", qreal),
+                ("twoQubitDephasing", qreal), ("twoQubitDepolarising", qreal)]
+
+
 def _ints(xs: Iterable[int]):
     xs = list(xs)
     return (C.c_int * max(1, len(xs)))(*xs), len(xs)
@@ -140,6 +151,8 @@ _SIGS = {
     "initStateFromAmps": (None, [Qureg, C.POINTER(qreal), C.POINTER(qreal)]),
     "copyStateToGPU": (None, [Qureg]),
     "copyStateFromGPU": (None, [Qureg]),
+    "copySubstateToGPU": (None, [Qureg, C.c_longlong, C.c_longlong]),
+    "copySubstateFromGPU": (None, [Qureg, C.c_longlong, C.c_longlong]),
     "getAmp": (Complex, [Qureg, C.c_longlong]),
     "getRealAmp": (qreal, [Qureg, C.c_longlong]),
     "getImagAmp": (qreal, [Qureg, C.c_longlong]),
@@ -280,6 +293,9 @@ _EXT_SIGS = {
     "qmxTimerSlotMs": (C.c_double, [C.c_int]),
     "qmxMaxAbsDiff": (qreal, [Qureg, Qureg]),
     "qmxRenormalise": (qreal, [Qureg, C.c_int]),
+    "qmxSaveStateCSV": (C.c_int, [Qureg, C.c_char_p]),
+    "qmxLoadStateCSV": (C.c_longlong, [Qureg, C.c_char_p]),
+    "reportState": (None, [Qureg]),
     "qmxCircuitCreate": (C.c_void_p, [C.c_int]),
     "qmxCircuitDestroy": (None, [C.c_void_p]),
     "qmxCircuitNumGates": (C.c_int, [C.c_void_p]),
@@ -292,7 +308,10 @@ _EXT_SIGS = {
                                            C.POINTER(qreal)]),
     "qmxCircuitFuse": (C.c_int, [C.c_void_p, C.c_int]),
     "qmxCircuitNumBlocks": (C.c_int, [C.c_void_p]),
-    "qmxCircuitApply": (None, [Qureg, C.c_void_p, C.c_int]),
+    "qmxCircuitApply": (qreal, [Qureg, C.c_void_p, C.c_int]),
+    "qmxCircuitAddOperation": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_int, qreal]),
+    "qmxCircuitSetNoiseAfterGate": (None, [C.c_void_p, NoiseAfterGate]),
+    "qmxApplyOperations": (qreal, [Qureg, C.POINTER(BasicOperation), C.c_int, C.POINTER(NoiseAfterGate), C.c_int]),
     "qmxSwapPlanCreate": (C.c_void_p, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int, C.c_int,
                                        C.POINTER(C.c_double), C.c_int]),
     "qmxSwapPlanDestroy": (None, [C.c_void_p]),

@@ -492,3 +492,73 @@ def test_controlled_pauli_rotations_match_dense(gpu):
         want = D.full_op(n, targs, U, controls=cs) @ want
     assert np.max(np.abs(gpu.state(q) - want)) < TOL
     gpu.destroy(q)
+
+
+# ---- regressions ---------------------------------------------------------------------------
+
+def _zz_hamil(n, terms, seed, with_x):
+    rng = np.random.default_rng(seed)
+    codes, coeffs = [], []
+    for _ in range(terms):
+        a, b = rng.choice(n, size=2, replace=False)
+        row = np.zeros(n, dtype=np.int32)
+        row[a] = row[b] = 3
+        codes.append(row)
+        coeffs.append(float(rng.uniform(0.9, 1.1)))  # rotation angle 2c: |cos(c)| >= 0.45
+    if with_x:
+        for k in range(n):
+            row = np.zeros(n, dtype=np.int32)
+            row[k] = 1
+            codes.append(row)
+            coeffs.append(0.3)
+    return np.array(coeffs), np.stack(codes)
+
+
+@pytest.mark.parametrize("density,n", [(False, 12), (False, 13), (True, 6)])
+@pytest.mark.parametrize("with_x", [False, True])
+def test_trotter_many_large_angle_rotations_fp32(gpu, cpuref, density, n, with_x):
+    """~100 ZZ terms at time 1 in one block: every rotation has |cos(theta/2)|
+    >= 1/2, so the tangent form would scale the tile by their product (2^-100
+    and below, past FP32's range). FP32 registers, basis / product start, against
+    the oracle."""
+    coeffs, codes = _zz_hamil(n, 100, seed=n, with_x=with_x)
+    outs = []
+    for lib, prec in ((gpu, QMX_PREC_FP32), (cpuref, None)):
+        q = lib.qureg(n, density=density, precision=prec)
+        if with_x:
+            lib.initPlusState(q)
+        else:
+            lib.initClassicalState(q, 5)
+        h = lib.hamil(coeffs, codes)
+        lib.call("applyTrotterCircuit", q, h, 1.0, 1, 1)
+        lib.lib.destroyPauliHamil(h)
+        outs.append(lib.state(q))
+        lib.destroy(q)
+    assert np.all(np.isfinite(outs[0]))
+    assert np.max(np.abs(outs[0] - outs[1])) < 2e-5
+
+
+@pytest.mark.parametrize("density,n", [(False, 3), (False, 7), (False, 13), (True, 3), (True, 6)])
+def test_prob_of_all_outcomes_matches_oracle(gpu, cpuref, density, n):
+    """Subsets of every size and order, low and high qubits, one pass each."""
+    rng = np.random.default_rng(n + 10 * density)
+    subsets = [[0], [n - 1], list(range(n)), list(reversed(range(n)))]
+    subsets += [[int(x) for x in rng.choice(n, size=int(rng.integers(1, n + 1)), replace=False)] for _ in range(6)]
+    regs = []
+    for lib in (gpu, cpuref):
+        q = lib.qureg(n, density=density)
+        if density:
+            pure = lib.qureg(n)
+            channel_circuit(lib, q, pure, n, 2, seed=3)
+            lib.destroy(pure)
+        else:
+            lib.initDebugState(q)
+            misc_ops(lib, q, n)
+        regs.append(q)
+    for qs in subsets:
+        a = gpu.calcProbOfAllOutcomes(regs[0], qs)
+        b = cpuref.calcProbOfAllOutcomes(regs[1], qs)
+        scale = max(1.0, float(np.max(np.abs(b))))
+        assert np.max(np.abs(a - b)) < 1e-12 * scale, qs
+    for lib, q in zip((gpu, cpuref), regs):
+        lib.destroy(q)

@@ -382,3 +382,55 @@ def test_cpuref_reduction_known_answers(cpuref):
     assert abs(cpuref.calcPurity(rho) - 1.0) < 1e-15
     cpuref.destroy(q)
     cpuref.destroy(rho)
+
+
+@pytest.mark.parametrize("density,n", [(False, 7), (True, 4)])
+def test_ops_oracle_equals_reference_loop(cpuref, reflib, density, n):
+    """QuantumLab's operation loop (oracle/ops_oracle.py) over every gate and
+    noise code, with after-gate noise on density registers, gives the same state
+    and collapse probabilities on the C restatement as on QuEST's CPU build."""
+    from oracle import ops_oracle as O
+    noise = dict(damping=0.02, dephasing=0.01, depolarising=0.01, mixPauli=0.01,
+                 twoQubitDepolarising=0.02, twoQubitDephasing=0.01) if density else None
+    ops = O.random_ops(n, 150, seed=n, noise=True)
+    for i in (40, 90):  # collapses with outcome 0 at positions where both libraries agree
+        ops.insert(i, (O.CC, [i % n], 0.0))
+    out = []
+    for lib in (cpuref, reflib):
+        q = lib.qureg(n, density=density)
+        lib.initPlusState(q)
+        prob = O.run_ops(lib, q, ops, noise)
+        out.append((lib.state(q), prob))
+        lib.destroy(q)
+    assert np.max(np.abs(out[0][0] - out[1][0])) < 1e-13
+    assert abs(out[0][1] - out[1][1]) < 1e-13
+
+
+def test_ops_oracle_gates_match_dense_operators(cpuref):
+    """Each gate code of the loop is the textbook operator (dense model)."""
+    from oracle import ops_oracle as O
+    n = 3
+    c, s = np.cos(0.35), np.sin(0.35)
+    rx = np.array([[c, -1j * s], [-1j * s, c]])
+    ry = np.array([[c, -s], [s, c]])
+    rz = np.diag([np.exp(-0.35j), np.exp(0.35j)])
+    p = np.diag([1, np.exp(0.7j)])
+    cases = [((O.H, [1], 0), D.full_op(n, [1], np.array([[1, 1], [1, -1]]) / np.sqrt(2))),
+             ((O.X, [2], 0), D.full_op(n, [2], D.X)), ((O.Y, [0], 0), D.full_op(n, [0], D.Y)),
+             ((O.Z, [1], 0), D.full_op(n, [1], D.Z)), ((O.P, [2], 0.7), D.full_op(n, [2], p)),
+             ((O.RX, [0], 0.7), D.full_op(n, [0], rx)), ((O.RY, [1], 0.7), D.full_op(n, [1], ry)),
+             ((O.RZ, [2], 0.7), D.full_op(n, [2], rz)),
+             ((O.CX, [0, 2], 0), D.full_op(n, [2], D.X, controls=[0])),
+             ((O.CY, [2, 1], 0), D.full_op(n, [1], D.Y, controls=[2])),
+             ((O.CZ, [1, 0], 0), D.full_op(n, [0], D.Z, controls=[1])),
+             ((O.CP, [0, 1], 0.7), D.full_op(n, [1], p, controls=[0])),
+             ((O.CRX, [1, 2], 0.7), D.full_op(n, [2], rx, controls=[1])),
+             ((O.CRY, [2, 0], 0.7), D.full_op(n, [0], ry, controls=[2])),
+             ((O.CRZ, [0, 1], 0.7), D.full_op(n, [1], rz, controls=[0]))]
+    psi = random_state(n, 3)
+    for op, M in cases:
+        q = cpuref.qureg(n)
+        cpuref.set_state(q, psi)
+        O.run_ops(cpuref, q, [op])
+        assert np.max(np.abs(cpuref.state(q) - M @ psi)) < 1e-14, op
+        cpuref.destroy(q)
