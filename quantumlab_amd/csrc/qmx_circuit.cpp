@@ -614,7 +614,7 @@ void qmxCircuitMultiQubitUnitary(qmxCircuit *c, int *targs, int numTargs, const 
         }
     try {
         qv::multiQubitUnitary(c->shape, o.mN, numTargs, __func__);
-    } catch (const qmx::InputError &) {
+    } catch (...) {  // also a caller's throwing error hook
         destroyComplexMatrixN(o.mN);
         throw;
     }

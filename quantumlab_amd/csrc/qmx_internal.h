@@ -7,7 +7,7 @@
 //          Interleaving makes every amplitude one 16 B (or 8 B) vector load,
 //          so a 1-target gate streams two arrays instead of the four SoA
 //          streams of the reference GPU backend (QuEST_gpu.cu:783-844).
-//   The host mirror of the Qureg stays SoA double, as callers expect.
+//   The host mirror of the Qureg is SoA qreal, as callers expect.
 #pragma once
 
 #include <cstdint>
@@ -76,8 +76,11 @@ void initDebug(QuregDev *q);
 void initSingleQubit(QuregDev *q, int qubit, int outcome);
 void initPureDensity(QuregDev *rho, const QuregDev *psi);  // rho = |psi><psi|
 void copyState(QuregDev *dst, const QuregDev *src);        // any precisions
+// host SoA arrays (double, or float for the QuEST_PREC 1 flavour) <-> device amplitudes
 void uploadAmps(QuregDev *q, long long start, const double *re, const double *im, long long count);
+void uploadAmps(QuregDev *q, long long start, const float *re, const float *im, long long count);
 void downloadAmps(const QuregDev *q, long long start, double *re, double *im, long long count);
+void downloadAmps(const QuregDev *q, long long start, float *re, float *im, long long count);
 void weightedSum(Cplx f1, const QuregDev *q1, Cplx f2, const QuregDev *q2, Cplx fo, QuregDev *out);
 void mixDensityMatrix(QuregDev *combine, double otherProb, const QuregDev *other);
 void scaleAll(QuregDev *q, double factor);

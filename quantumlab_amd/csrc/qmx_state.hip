@@ -67,23 +67,23 @@ __global__ void kCopy(Amp<TD> *d, const Amp<TS> *s, long long n) {
         if (i < n) d[i] = Amp<TD>{(TD)s[i].x, (TD)s[i].y};
 }
 
-template <typename T>
-__global__ void kInterleave(Amp<T> *a, const double *re, const double *im, long long n) {
+template <typename T, typename H>
+__global__ void kInterleave(Amp<T> *a, const H *re, const H *im, long long n) {
     long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) a[i] = Amp<T>{(T)re[i], (T)im[i]};
 }
 
-template <typename T>
-__global__ void kDeinterleave(const Amp<T> *a, double *re, double *im, long long n) {
+template <typename T, typename H>
+__global__ void kDeinterleave(const Amp<T> *a, H *re, H *im, long long n) {
     long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) {
             Amp<T> v = a[i];
-            re[i] = (double)v.x;
-            im[i] = (double)v.y;
+            re[i] = (H)v.x;
+            im[i] = (H)v.y;
         }
 }
 
@@ -205,16 +205,18 @@ void copyState(QuregDev *dst, const QuregDev *src) {
     checkLaunch("copyState");
 }
 
-void uploadAmps(QuregDev *q, long long start, const double *re, const double *im, long long count) {
+// Host SoA arrays of H (double or float) <-> the interleaved device amplitudes,
+// through the staging buffer in chunks of kStageAmps.
+template <typename H>
+void uploadT(QuregDev *q, long long start, const H *re, const H *im, long long count) {
     std::lock_guard<std::mutex> lk(g_stageMu);
-    double *s = stage();
+    H *s = reinterpret_cast<H *>(stage());
     for (long long off = 0; off < count; off += kStageAmps) {
         long long m = std::min(kStageAmps, count - off);
-        QMX_HIP_CHECK(hipMemcpyAsync(s, re + off, m * sizeof(double), hipMemcpyHostToDevice, hstream()));
-        QMX_HIP_CHECK(hipMemcpyAsync(s + kStageAmps, im + off, m * sizeof(double),
-                                     hipMemcpyHostToDevice, hstream()));
+        QMX_HIP_CHECK(hipMemcpyAsync(s, re + off, m * sizeof(H), hipMemcpyHostToDevice, hstream()));
+        QMX_HIP_CHECK(hipMemcpyAsync(s + kStageAmps, im + off, m * sizeof(H), hipMemcpyHostToDevice, hstream()));
         QMX_DISPATCH(q->prec, T, {
-            hipLaunchKernelGGL(kInterleave<T>, dim3(gridFor(m, 4)), dim3(kBlock), 0, hstream(),
+            hipLaunchKernelGGL((kInterleave<T, H>), dim3(gridFor(m, 4)), dim3(kBlock), 0, hstream(),
                                (Amp<T> *)q->amps + start + off, s, s + kStageAmps, m);
         });
         checkLaunch("uploadAmps");
@@ -222,21 +224,34 @@ void uploadAmps(QuregDev *q, long long start, const double *re, const double *im
     QMX_HIP_CHECK(hipStreamSynchronize(hstream()));
 }
 
-void downloadAmps(const QuregDev *q, long long start, double *re, double *im, long long count) {
+template <typename H>
+void downloadT(const QuregDev *q, long long start, H *re, H *im, long long count) {
     std::lock_guard<std::mutex> lk(g_stageMu);
-    double *s = stage();
+    H *s = reinterpret_cast<H *>(stage());
     for (long long off = 0; off < count; off += kStageAmps) {
         long long m = std::min(kStageAmps, count - off);
         QMX_DISPATCH(q->prec, T, {
-            hipLaunchKernelGGL(kDeinterleave<T>, dim3(gridFor(m, 4)), dim3(kBlock), 0, hstream(),
+            hipLaunchKernelGGL((kDeinterleave<T, H>), dim3(gridFor(m, 4)), dim3(kBlock), 0, hstream(),
                                (const Amp<T> *)q->amps + start + off, s, s + kStageAmps, m);
         });
         checkLaunch("downloadAmps");
-        QMX_HIP_CHECK(hipMemcpyAsync(re + off, s, m * sizeof(double), hipMemcpyDeviceToHost, hstream()));
-        QMX_HIP_CHECK(hipMemcpyAsync(im + off, s + kStageAmps, m * sizeof(double),
-                                     hipMemcpyDeviceToHost, hstream()));
+        QMX_HIP_CHECK(hipMemcpyAsync(re + off, s, m * sizeof(H), hipMemcpyDeviceToHost, hstream()));
+        QMX_HIP_CHECK(hipMemcpyAsync(im + off, s + kStageAmps, m * sizeof(H), hipMemcpyDeviceToHost, hstream()));
         QMX_HIP_CHECK(hipStreamSynchronize(hstream()));
     }
+}
+
+void uploadAmps(QuregDev *q, long long start, const double *re, const double *im, long long count) {
+    uploadT(q, start, re, im, count);
+}
+void uploadAmps(QuregDev *q, long long start, const float *re, const float *im, long long count) {
+    uploadT(q, start, re, im, count);
+}
+void downloadAmps(const QuregDev *q, long long start, double *re, double *im, long long count) {
+    downloadT(q, start, re, im, count);
+}
+void downloadAmps(const QuregDev *q, long long start, float *re, float *im, long long count) {
+    downloadT(q, start, re, im, count);
 }
 
 void weightedSum(Cplx f1, const QuregDev *q1, Cplx f2, const QuregDev *q2, Cplx fo, QuregDev *out) {

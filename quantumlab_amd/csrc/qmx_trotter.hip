@@ -243,7 +243,8 @@ struct Packed {
     std::vector<double> theta, phi;
 };
 
-bool pack(int numJobs, const PauliHamil *hamils, const double *times, const double *theta, const double *phi,
+template <typename R>
+bool pack(int numJobs, const PauliHamil *hamils, const R *times, const R *theta, const R *phi,
           const int *orders, Packed &pk, const char *caller) {
     std::vector<int> stageOff(64, -1), stageCnt(64, 0);
     long long angle = 0;

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "QuEST.h"
@@ -43,6 +44,38 @@ void *qmxTryDevAlloc(size_t bytes);
     }
 
 namespace {
+
+// A qreal array read by code that takes double: the array itself when qreal is
+// double (QuEST_PREC 2), else an FP64 copy (QuEST_PREC 1). NULL stays NULL.
+class Doubles {
+    std::vector<double> copy_;
+    const double *p_ = nullptr;
+
+public:
+    Doubles(const qreal *x, size_t n) {
+        if (!x) return;
+        if constexpr (std::is_same<qreal, double>::value) {
+            p_ = reinterpret_cast<const double *>(x);
+        } else {
+            copy_.assign(x, x + n);
+            p_ = copy_.data();
+        }
+    }
+    const double *get() const { return p_; }
+};
+
+// Device copies of a DiagonalOp hold FP64 elements in both flavours (the
+// kernels read double); deviceOperator.real / .imag point at them.
+double *devDiag(qreal *p) { return reinterpret_cast<double *>(p); }
+
+// createQureg / createDensityQureg store amplitudes at qreal's precision, as the
+// reference's GPU backend does; createQuregEx chooses explicitly
+constexpr int kDefaultPrec = sizeof(qreal) == sizeof(float) ? qmx::FP32 : qmx::FP64;
+
+void diagToDevice(qreal *dev, const qreal *host, size_t n) {
+    Doubles d(host, n);
+    qmx::hostToDevice(devDiag(dev), d.get(), n * sizeof(double));
+}
 
 uint64_t bitMask(const int *qubits, int n) {
     uint64_t m = 0;
@@ -503,9 +536,8 @@ Qureg createQuregEx(int numQubits, QuESTEnv env, int precision) {
     Qureg empty;
     std::memset(&empty, 0, sizeof empty);
     API_TRY
-    (void)env;
     qv::precision(precision, __func__);
-    qv::numQubitsInQureg(numQubits, __func__);
+    qv::numQubitsInQureg(numQubits, __func__, env.numRanks);
     return makeQureg(numQubits, 0, numQubits, precision, __func__);
     API_CATCH(empty)
 }
@@ -514,9 +546,8 @@ Qureg createDensityQuregEx(int numQubits, QuESTEnv env, int precision) {
     Qureg empty;
     std::memset(&empty, 0, sizeof empty);
     API_TRY
-    (void)env;
     qv::precision(precision, __func__);
-    qv::numQubitsInQureg(2 * numQubits, __func__);
+    qv::numQubitsInQureg(2 * numQubits, __func__, env.numRanks);
     return makeQureg(2 * numQubits, 1, numQubits, precision, __func__);
     API_CATCH(empty)
 }
@@ -525,9 +556,8 @@ Qureg createQureg(int numQubits, QuESTEnv env) {
     Qureg empty;
     std::memset(&empty, 0, sizeof empty);
     API_TRY
-    (void)env;
-    qv::numQubitsInQureg(numQubits, __func__);
-    return makeQureg(numQubits, 0, numQubits, qmx::FP64, __func__);
+    qv::numQubitsInQureg(numQubits, __func__, env.numRanks);
+    return makeQureg(numQubits, 0, numQubits, kDefaultPrec, __func__);
     API_CATCH(empty)
 }
 
@@ -535,9 +565,8 @@ Qureg createDensityQureg(int numQubits, QuESTEnv env) {
     Qureg empty;
     std::memset(&empty, 0, sizeof empty);
     API_TRY
-    (void)env;
-    qv::numQubitsInQureg(2 * numQubits, __func__);
-    return makeQureg(2 * numQubits, 1, numQubits, qmx::FP64, __func__);
+    qv::numQubitsInQureg(2 * numQubits, __func__, env.numRanks);
+    return makeQureg(2 * numQubits, 1, numQubits, kDefaultPrec, __func__);
     API_CATCH(empty)
 }
 
@@ -716,7 +745,7 @@ PauliHamil createPauliHamilFromFile(char *fn) {
                 parsed.pauliCodes[t * numQubits + q] = (enum pauliOpType)code;
             }
         }
-    } catch (const qmx::InputError &) {
+    } catch (...) {  // also a caller's throwing error hook
         destroyPauliHamil(parsed);
         throw;
     }
@@ -1465,7 +1494,9 @@ qreal calcProbOfOutcome(Qureg qureg, int measureQubit, int outcome) {
 void calcProbOfAllOutcomes(qreal *outcomeProbs, Qureg qureg, int *qubits, int numQubits) {
     API_TRY
     qv::multiTargets(qureg, qubits, numQubits, __func__);
-    qmx::probOfAllOutcomes(D(qureg), qubits, numQubits, outcomeProbs);
+    std::vector<double> p((size_t)1 << numQubits);
+    qmx::probOfAllOutcomes(D(qureg), qubits, numQubits, p.data());
+    for (size_t o = 0; o < p.size(); ++o) outcomeProbs[o] = (qreal)p[o];
     API_CATCH_VOID
 }
 
@@ -1581,8 +1612,10 @@ int measureWithStats(Qureg qureg, int measureQubit, qreal *outcomeProb) {
     API_TRY
     qv::target(qureg, measureQubit, __func__);
     double p0 = probOfOutcome(qureg, measureQubit, 0);
-    int outcome = generateMeasurementOutcome(p0, outcomeProb);
-    collapseKnown(qureg, measureQubit, outcome, *outcomeProb);
+    double prob = 0;
+    int outcome = generateMeasurementOutcome(p0, &prob);
+    collapseKnown(qureg, measureQubit, outcome, prob);
+    *outcomeProb = (qreal)prob;
     return outcome;
     API_CATCH(0)
 }
@@ -1805,7 +1838,7 @@ DiagonalOp createDiagonalOp(int numQubits, QuESTEnv env) {
     (void)env;
     DiagonalOp op = emptyDiagonalOp();
     API_TRY
-    qv::numQubitsInDiagOp(numQubits, __func__);
+    qv::numQubitsInDiagOp(numQubits, __func__, env.numRanks);
     qmx::runtimeInit();
     const long long n = numQubits < 62 ? 1LL << numQubits : 0;
     op.numQubits = numQubits;
@@ -1820,17 +1853,17 @@ DiagonalOp createDiagonalOp(int numQubits, QuESTEnv env) {
         op = emptyDiagonalOp();
         qv::raise(qv::message(qv::E_DIAGONAL_OP_NOT_ALLOCATED), __func__);
     }
-    auto *dev = static_cast<qreal *>(qmxTryDevAlloc((size_t)n * 2 * sizeof(qreal)));
+    auto *dev = static_cast<double *>(qmxTryDevAlloc((size_t)n * 2 * sizeof(double)));
     if (!dev) {
         std::free(op.real);
         std::free(op.imag);
         op = emptyDiagonalOp();
         qv::raise(qv::message(qv::E_DIAGONAL_OP_NOT_ALLOCATED_ON_GPU), __func__);
     }
-    op.deviceOperator.real = dev;
-    op.deviceOperator.imag = dev + n;
-    qmx::hostToDevice(op.deviceOperator.real, op.real, (size_t)n * sizeof(qreal));
-    qmx::hostToDevice(op.deviceOperator.imag, op.imag, (size_t)n * sizeof(qreal));
+    op.deviceOperator.real = reinterpret_cast<qreal *>(dev);
+    op.deviceOperator.imag = reinterpret_cast<qreal *>(dev + n);
+    diagToDevice(op.deviceOperator.real, op.real, (size_t)n);
+    diagToDevice(op.deviceOperator.imag, op.imag, (size_t)n);
     return op;
     API_CATCH(op)
 }
@@ -1848,9 +1881,8 @@ void destroyDiagonalOp(DiagonalOp op, QuESTEnv env) {
 void syncDiagonalOp(DiagonalOp op) {
     API_TRY
     qv::diagOpInit(op, __func__);
-    const size_t bytes = (size_t)op.numElemsPerChunk * sizeof(qreal);
-    qmx::hostToDevice(op.deviceOperator.real, op.real, bytes);
-    qmx::hostToDevice(op.deviceOperator.imag, op.imag, bytes);
+    diagToDevice(op.deviceOperator.real, op.real, (size_t)op.numElemsPerChunk);
+    diagToDevice(op.deviceOperator.imag, op.imag, (size_t)op.numElemsPerChunk);
     API_CATCH_VOID
 }
 
@@ -1858,8 +1890,8 @@ void syncDiagonalOp(DiagonalOp op) {
 static void setDiagElems(DiagonalOp op, long long start, const qreal *re, const qreal *im, long long n) {
     std::memcpy(op.real + start, re, (size_t)n * sizeof(qreal));
     std::memcpy(op.imag + start, im, (size_t)n * sizeof(qreal));
-    qmx::hostToDevice(op.deviceOperator.real + start, re, (size_t)n * sizeof(qreal));
-    qmx::hostToDevice(op.deviceOperator.imag + start, im, (size_t)n * sizeof(qreal));
+    diagToDevice(reinterpret_cast<qreal *>(devDiag(op.deviceOperator.real) + start), re, (size_t)n);
+    diagToDevice(reinterpret_cast<qreal *>(devDiag(op.deviceOperator.imag) + start), im, (size_t)n);
 }
 
 void initDiagonalOp(DiagonalOp op, qreal *real, qreal *imag) {
@@ -1883,7 +1915,7 @@ static void diagFromHamil(DiagonalOp op, const PauliHamil &h) {
         for (int q = 0; q < h.numQubits; ++q)
             if (h.pauliCodes[(size_t)t * h.numQubits + q] == PAULI_Z) z[t] |= 1ull << q;
     std::vector<double> c(h.termCoeffs, h.termCoeffs + h.numSumTerms);
-    qmx::diagonalFromZHamil(op.deviceOperator.real, op.deviceOperator.imag, op.numQubits, z.data(), c.data(),
+    qmx::diagonalFromZHamil(devDiag(op.deviceOperator.real), devDiag(op.deviceOperator.imag), op.numQubits, z.data(), c.data(),
                             h.numSumTerms);
     const size_t bytes = (size_t)op.numElemsPerChunk * sizeof(qreal);
     qmx::deviceToHost(op.real, op.deviceOperator.real, bytes);
@@ -1918,7 +1950,7 @@ DiagonalOp createDiagonalOpFromPauliHamilFile(char *fn, QuESTEnv env) {
 void applyDiagonalOp(Qureg qureg, DiagonalOp op) {
     API_TRY
     qv::diagonalOp(qureg, op, __func__);
-    qmx::applyDiagonal(D(qureg), op.deviceOperator.real, op.deviceOperator.imag);
+    qmx::applyDiagonal(D(qureg), devDiag(op.deviceOperator.real), devDiag(op.deviceOperator.imag));
     API_CATCH_VOID
 }
 
@@ -1926,7 +1958,7 @@ Complex calcExpecDiagonalOp(Qureg qureg, DiagonalOp op) {
     Complex z{0, 0};
     API_TRY
     qv::diagonalOp(qureg, op, __func__);
-    const Cplx r = qmx::expecDiagonal(D(qureg), op.deviceOperator.real, op.deviceOperator.imag);
+    const Cplx r = qmx::expecDiagonal(D(qureg), devDiag(op.deviceOperator.real), devDiag(op.deviceOperator.imag));
     z.real = r.re;
     z.imag = r.im;
     return z;
@@ -1939,21 +1971,23 @@ Complex calcExpecDiagonalOp(Qureg qureg, DiagonalOp op) {
 static void phaseFuncImpl(const Qureg &qureg, int *qubits, int numQubits, int encoding, qreal *coeffs,
                           qreal *exponents, int numTerms, long long *overrideInds, qreal *overridePhases,
                           int numOverrides, const char *caller) {
+    const Doubles c(coeffs, (size_t)std::max(0, numTerms)), e(exponents, (size_t)std::max(0, numTerms));
+    const Doubles ph(overridePhases, (size_t)std::max(0, numOverrides));
     qv::phaseMultiQubits(qureg, qubits, numQubits, caller);
     qv::bitEncoding(numQubits, encoding, caller);
     if (overrideInds) qv::phaseFuncOverrides(numQubits, encoding, overrideInds, numOverrides, caller);
-    qv::phaseFuncTerms(numQubits, encoding, exponents, numTerms, overrideInds, numOverrides, caller);
+    qv::phaseFuncTerms(numQubits, encoding, e.get(), numTerms, overrideInds, numOverrides, caller);
     qmx::PhaseFuncArgs a{};
     a.numRegs = 1;
     a.encoding = encoding;
     a.mode = 0;
     a.qubits = qubits;
     a.numQubitsPerReg = &numQubits;
-    a.coeffs = coeffs;
-    a.exponents = exponents;
+    a.coeffs = c.get();
+    a.exponents = e.get();
     a.numTermsPerReg = &numTerms;
     a.overrideInds = overrideInds;
-    a.overridePhases = overridePhases;
+    a.overridePhases = ph.get();
     a.numOverrides = overrideInds ? numOverrides : 0;
     phaseFuncPasses(qureg, a);
 }
@@ -1962,8 +1996,12 @@ static void multiVarImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, 
                          qreal *coeffs, qreal *exponents, int *numTermsPerReg, long long *overrideInds,
                          qreal *overridePhases, int numOverrides, const char *caller) {
     qv::qubitSubregs(qureg, qubits, numQubitsPerReg, numRegs, caller);
+    size_t totalTerms = 0;
+    for (int r = 0; r < numRegs; ++r) totalTerms += (size_t)std::max(0, numTermsPerReg[r]);
+    const Doubles c(coeffs, totalTerms), e(exponents, totalTerms);
+    const Doubles ph(overridePhases, (size_t)std::max(0, numOverrides));
     qv::multiRegBitEncoding(numQubitsPerReg, numRegs, encoding, caller);
-    qv::multiVarPhaseFuncTerms(numRegs, encoding, exponents, numTermsPerReg, caller);
+    qv::multiVarPhaseFuncTerms(numRegs, encoding, e.get(), numTermsPerReg, caller);
     if (overrideInds)
         qv::multiVarPhaseFuncOverrides(numQubitsPerReg, numRegs, encoding, overrideInds, numOverrides, caller);
     qmx::PhaseFuncArgs a{};
@@ -1972,11 +2010,11 @@ static void multiVarImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, 
     a.mode = 0;
     a.qubits = qubits;
     a.numQubitsPerReg = numQubitsPerReg;
-    a.coeffs = coeffs;
-    a.exponents = exponents;
+    a.coeffs = c.get();
+    a.exponents = e.get();
     a.numTermsPerReg = numTermsPerReg;
     a.overrideInds = overrideInds;
-    a.overridePhases = overridePhases;
+    a.overridePhases = ph.get();
     a.numOverrides = overrideInds ? numOverrides : 0;
     phaseFuncPasses(qureg, a);
 }
@@ -1984,6 +2022,8 @@ static void multiVarImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, 
 static void namedImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, int numRegs, int encoding, int code,
                       qreal *params, int numParams, long long *overrideInds, qreal *overridePhases, int numOverrides,
                       const char *caller) {
+    const Doubles pr(params, (size_t)std::max(0, numParams));
+    const Doubles ph(overridePhases, (size_t)std::max(0, numOverrides));
     qv::qubitSubregs(qureg, qubits, numQubitsPerReg, numRegs, caller);
     qv::multiRegBitEncoding(numQubitsPerReg, numRegs, encoding, caller);
     qv::phaseFuncName(code, numRegs, numParams, caller);
@@ -1996,10 +2036,10 @@ static void namedImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, int
     a.funcCode = code;
     a.qubits = qubits;
     a.numQubitsPerReg = numQubitsPerReg;
-    a.params = params;
+    a.params = pr.get();
     a.numParams = params ? numParams : 0;
     a.overrideInds = overrideInds;
-    a.overridePhases = overridePhases;
+    a.overridePhases = ph.get();
     a.numOverrides = overrideInds ? numOverrides : 0;
     phaseFuncPasses(qureg, a);
 }
@@ -2129,7 +2169,8 @@ void qmxAmplitudeEncodeReal(Qureg qureg, const qreal *vec, int numQubits) {
     if (numQubits < 1 || numQubits > 12 || numQubits > qureg.numQubitsRepresented)
         qv::raise("Invalid number of qubits to encode. Must be >0, <=12 and <=numQubits.", __func__);
     if (!vec) qv::raise("The vector is null.", __func__);
-    qmx::amplitudeEncode(D(qureg), vec, numQubits);
+    Doubles v(vec, (size_t)1 << numQubits);
+    qmx::amplitudeEncode(D(qureg), v.get(), numQubits);
     API_CATCH_VOID
 }
 

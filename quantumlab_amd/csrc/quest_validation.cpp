@@ -165,6 +165,11 @@ const char *const kMessages[] = {
     "cannot be (efficiently) checked that all negative indices were overriden. One must instead call "
     "applyPhaseFuncOverrides() once for each register, so that each register's negative indices can be "
     "overriden, independent of the indices of all other registers.",
+    /* E_DISTRIB_QUREG_TOO_SMALL */
+    "Too few qubits. The created qureg must have at least one amplitude per node used in distributed simulation.",
+    /* E_DISTRIB_DIAG_OP_TOO_SMALL */
+    "Too few qubits. The created DiagonalOp must contain at least one element per node used in distributed "
+    "simulation.",
 };
 static_assert(sizeof(kMessages) / sizeof(kMessages[0]) == E_NUM_CODES, "message table out of sync");
 
@@ -288,11 +293,15 @@ static bool uniqueQubits(const int *q, int n) {
 
 // ---- checks -------------------------------------------------------------------
 
-void numQubitsInQureg(int numQubits, const char *f) {
+// numRanks: QuESTEnv.numRanks; a register needs at least one amplitude per rank
+// (QuEST_validation.c:383-393; compared in 64 bits here, so that registers of
+// 31 qubits and more pass)
+void numQubitsInQureg(int numQubits, const char *f, int numRanks) {
     check(numQubits > 0, E_INVALID_NUM_CREATE_QUBITS, f);
     unsigned maxQ = 0;
     for (size_t v = SIZE_MAX; v >>= 1;) ++maxQ;
     check(numQubits <= (int)maxQ, E_NUM_AMPS_EXCEED_TYPE, f);
+    check(numQubits >= 62 || (1LL << numQubits) >= (long long)numRanks, E_DISTRIB_QUREG_TOO_SMALL, f);
 }
 void numQubitsInMatrix(int numQubits, const char *f) { check(numQubits > 0, E_INVALID_NUM_QUBITS, f); }
 void stateIndex(const Qureg &q, long long ind, const char *f) {
@@ -493,9 +502,10 @@ void precision(int p, const char *f) { check(p == 0 || p == 1, E_INVALID_PRECISI
 
 
 // ---- diagonal operators ----------------------------------------------------------
-void numQubitsInDiagOp(int numQubits, const char *f) {
+void numQubitsInDiagOp(int numQubits, const char *f, int numRanks) {
     check(numQubits > 0, E_INVALID_NUM_CREATE_QUBITS, f);
     check(numQubits <= 63, E_NUM_AMPS_EXCEED_TYPE, f);
+    check(numQubits >= 62 || (1LL << numQubits) >= (long long)numRanks, E_DISTRIB_DIAG_OP_TOO_SMALL, f);
 }
 void numElems(const DiagonalOp &op, long long start, long long n, const char *f) {
     const long long mx = 1LL << op.numQubits;

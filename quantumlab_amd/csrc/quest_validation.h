@@ -95,6 +95,8 @@ enum ErrorCode {
     E_FRACTIONAL_EXPONENT_WITHOUT_NEG_OVERRIDE,
     E_NEGATIVE_EXPONENT_MULTI_VAR,
     E_FRACTIONAL_EXPONENT_MULTI_VAR,
+    E_DISTRIB_QUREG_TOO_SMALL,
+    E_DISTRIB_DIAG_OP_TOO_SMALL,
     E_NUM_CODES
 };
 
@@ -110,7 +112,7 @@ void checkFmt(bool ok, ErrorCode code, const char *func, const char *arg);
 void checkFmtCode(bool ok, ErrorCode code, const char *func, const char *arg, int code2);
 bool validPauliCode(int code);
 
-void numQubitsInQureg(int numQubits, const char *f);
+void numQubitsInQureg(int numQubits, const char *f, int numRanks = 1);
 void numQubitsInMatrix(int numQubits, const char *f);
 void stateIndex(const Qureg &q, long long ind, const char *f);
 void ampIndex(const Qureg &q, long long ind, const char *f);
@@ -163,7 +165,7 @@ void matchingQuregPauliHamilDims(const Qureg &q, const PauliHamil &h, const char
 void trotterParams(int order, int reps, const char *f);
 
 // diagonal operators (QuEST_validation.c:399-409, 436-441, 773-819)
-void numQubitsInDiagOp(int numQubits, const char *f);
+void numQubitsInDiagOp(int numQubits, const char *f, int numRanks = 1);
 void numElems(const DiagonalOp &op, long long start, long long n, const char *f);
 void diagOpInit(const DiagonalOp &op, const char *f);
 void diagonalOp(const Qureg &q, const DiagonalOp &op, const char *f);

@@ -31,7 +31,7 @@ from typing import List, Sequence
 
 import numpy as np
 
-from .quest import QuEST, cplx, matrix2, qreal
+from .quest import QuEST
 
 
 def haar_unitary(rng: np.random.Generator, dim: int) -> np.ndarray:
@@ -110,7 +110,7 @@ def apply_gates(qe: QuEST, q, gates: Sequence[tuple]):
     for g in gates:
         kind = g[0]
         if kind == "cu":
-            qe.call("compactUnitary", q, g[1], cplx(g[2]), cplx(g[3]))
+            qe.call("compactUnitary", q, g[1], qe.t.cplx(g[2]), qe.t.cplx(g[3]))
         elif kind == "cnot":
             qe.call("controlledNot", q, g[1], g[2])
         elif kind == "rz":
@@ -136,21 +136,21 @@ def build_circuit(qe: QuEST, n: int, gates: Sequence[tuple]):
     for g in gates:
         kind = g[0]
         if kind == "cu":
-            lib.qmxCircuitCompactUnitary(c, g[1], cplx(g[2]), cplx(g[3]))
+            lib.qmxCircuitCompactUnitary(c, g[1], qe.t.cplx(g[2]), qe.t.cplx(g[3]))
         elif kind == "cnot":
             lib.qmxCircuitControlledNot(c, g[1], g[2])
         elif kind == "rz":
             lib.qmxCircuitRotateZ(c, g[1], g[2])
         elif kind == "mcu":
             arr = (C.c_int * len(g[1]))(*g[1])
-            lib.qmxCircuitMultiControlledUnitary(c, arr, len(g[1]), g[2], matrix2(g[3]))
+            lib.qmxCircuitMultiControlledUnitary(c, arr, len(g[1]), g[2], qe.t.matrix2(g[3]))
         elif kind == "mqu":
             arr = (C.c_int * len(g[1]))(*g[1])
             u = np.ascontiguousarray(g[2], dtype=np.complex128)
-            re = np.ascontiguousarray(u.real).ravel()
-            im = np.ascontiguousarray(u.imag).ravel()
-            lib.qmxCircuitMultiQubitUnitary(c, arr, len(g[1]), re.ctypes.data_as(C.POINTER(qreal)),
-                                            im.ctypes.data_as(C.POINTER(qreal)))
+            re = qe.reals(u.real).ravel()
+            im = qe.reals(u.imag).ravel()
+            lib.qmxCircuitMultiQubitUnitary(c, arr, len(g[1]), re.ctypes.data_as(qe.real_ptr),
+                                            im.ctypes.data_as(qe.real_ptr))
         else:
             raise ValueError(f"unknown gate {kind}")
     return c

@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from quantumlab_amd import workloads as W
-from quantumlab_amd.quest import QuEST, Vector, cplx, matrix2, matrix4
+from quantumlab_amd.quest import QuEST
 
 
 def ints(xs):
@@ -46,6 +46,7 @@ def pauli_rotation_cases(n: int, seed: int):
 def misc_ops(qe: QuEST, q, n: int):
     """A fixed sequence over most of the gate API (needs n >= 3)."""
     rng = np.random.default_rng(11)
+    t = qe.t  # struct types of this library's qreal
     a, b = W.compact_pair(rng)
     u2 = W.haar_unitary(rng, 2)
     u2b = W.haar_unitary(rng, 2)
@@ -65,16 +66,16 @@ def misc_ops(qe: QuEST, q, n: int):
     qe.call("rotateX", q, 0, 0.1)
     qe.call("rotateY", q, 1, 0.2)
     qe.call("rotateZ", q, 2, 0.3)
-    qe.call("rotateAroundAxis", q, 1, 0.4, Vector(1.0, 2.0, 3.0))
+    qe.call("rotateAroundAxis", q, 1, 0.4, t.Vector(1.0, 2.0, 3.0))
     qe.call("controlledRotateX", q, 0, 1, 0.5)
     qe.call("controlledRotateY", q, 1, 2, 0.6)
     qe.call("controlledRotateZ", q, 2, 0, 0.7)
-    qe.call("controlledCompactUnitary", q, 0, 2, cplx(a), cplx(b))
-    qe.call("unitary", q, 1, matrix2(u2))
-    qe.call("controlledUnitary", q, 2, 0, matrix2(u2b))
-    qe.call("multiStateControlledUnitary", q, ints([0, 2]), ints([0, 1]), 2, 1, matrix2(u2c))
+    qe.call("controlledCompactUnitary", q, 0, 2, t.cplx(a), t.cplx(b))
+    qe.call("unitary", q, 1, t.matrix2(u2))
+    qe.call("controlledUnitary", q, 2, 0, t.matrix2(u2b))
+    qe.call("multiStateControlledUnitary", q, ints([0, 2]), ints([0, 1]), 2, 1, t.matrix2(u2c))
     qe.call("swapGate", q, 0, 2)
-    qe.call("twoQubitUnitary", q, 0, 2, matrix4(u4))
+    qe.call("twoQubitUnitary", q, 0, 2, t.matrix4(u4))
     qe.call("multiControlledMultiQubitNot", q, ints([1]), 1, ints([0, 2]), 2)
     qe.call("multiQubitNot", q, ints([0, 1]), 2)
     qe.multiRotateZ(q, [0, 1, 2], 0.9)

@@ -2,7 +2,7 @@
 backend (oracle/_ref/libquest_ref.so, compiled by oracle/Makefile from the
 reference sources in the development container).
 
-Run from the repository root:  python tests/golden/make_golden.py
+Run from the repository root:  python tests/golden/make_golden.py [--f32]
 The fixtures are inputs (seeds, gate-list parameters reproduced by
 quantumlab_amd.workloads) and the reference's outputs; no reference code is
 stored. A manifest with sha256 of each fixture is written next to them.
@@ -33,6 +33,90 @@ def save(name, **arrays):
     path = os.path.join(OUT, name + ".npz")
     np.savez_compressed(path, **arrays)
     return path
+
+
+REF32 = os.path.join(ROOT, "oracle", "_ref", "libquest_ref_f32.so")
+
+
+def main_f32():
+    """Fixtures of QuEST's default precision (QuEST_PREC=1, qreal = float), from
+    the same sources built at that precision, for the float flavour of the
+    library (libquest_mi355x_f32.so). Inputs are generated in FP64 and rounded
+    to float by the binding, exactly as the float library receives them."""
+    if not os.path.exists(REF32):
+        raise SystemExit(f"{REF32} missing: run `make -C oracle ref` in the development container")
+    qe = QuEST(REF32, qreal=C.c_float)
+    written = []
+
+    n, layers, seed = 12, 20, 21
+    q = qe.qureg(n)
+    W.apply_gates(qe, q, W.brick_circuit(n, layers, seed, extra_every=5))
+    written.append(save("f32_brick12_extras", n=n, layers=layers, seed=seed, state=qe.state(q),
+                        total=qe.calcTotalProb(q),
+                        prob0=np.array([qe.calcProbOfOutcome(q, k, 0) for k in range(n)])))
+    qe.destroy(q)
+
+    for density, nq in ((False, 5), (True, 3)):
+        q = qe.qureg(nq, density=density)
+        qe.initDebugState(q)
+        misc_ops(qe, q, nq)
+        written.append(save(f"f32_misc_{'dm' if density else 'sv'}{nq}", n=nq, state=qe.state(q)))
+        qe.destroy(q)
+
+    n, layers, seed = 5, 6, 23
+    rho = qe.qureg(n, density=True)
+    pure = qe.qureg(n)
+    channel_circuit(qe, rho, pure, n, layers, seed)
+    written.append(save("f32_density5_channels", n=n, layers=layers, seed=seed, state=qe.state(rho),
+                        purity=qe.calcPurity(rho), total=qe.calcTotalProb(rho),
+                        prob0=np.array([qe.calcProbOfOutcome(rho, k, 0) for k in range(n)]),
+                        fidelity=qe.calcFidelity(rho, pure)))
+    qe.destroy(rho)
+    qe.destroy(pure)
+
+    nq = 8
+    coeffs, codes = random_hamil(nq, 20, seed=25)
+    h = qe.hamil(coeffs, codes)
+    states = {}
+    for order in (1, 2, 4):
+        q = qe.qureg(nq)
+        qe.set_state(q, random_state(nq, seed=26))
+        qe.applyTrotterCircuit(q, h, 0.7, order, 3)
+        states[f"order{order}"] = qe.state(q)
+        qe.destroy(q)
+    written.append(save("f32_trotter8", n=nq, coeffs=coeffs, codes=codes, time=0.7, reps=3, **states))
+
+    nq = 10
+    qe.seed([4321, 8765])
+    q = qe.qureg(nq)
+    qe.set_state(q, random_state(nq, seed=27))
+    outcomes, probs = [], []
+    for k in list(range(nq)) + list(range(nq)):
+        o, p = qe.measureWithStats(q, k)
+        outcomes.append(o)
+        probs.append(p)
+        if k == nq - 1:
+            qe.set_state(q, random_state(nq, seed=28))
+    written.append(save("f32_measure10", n=nq, seeds=np.array([4321, 8765]), outcomes=np.array(outcomes),
+                        probs=np.array(probs), state=qe.state(q)))
+    qe.destroy(q)
+    return written
+
+
+def write_manifest(written):
+    path = os.path.join(OUT, "MANIFEST.json")
+    manifest = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            manifest = json.load(f).get("files", {})
+    for p in written:
+        with open(p, "rb") as f:
+            manifest[os.path.basename(p)] = hashlib.sha256(f.read()).hexdigest()
+    with open(path, "w") as f:
+        json.dump({"generator": "tests/golden/make_golden.py",
+                   "reference": "QuEST 3.4 CPU backend (Code/QuEST340/QuEST), -fopenmp; QuEST_PREC=2, "
+                                "and QuEST_PREC=1 for the f32_* fixtures",
+                   "files": manifest}, f, indent=1, sort_keys=True)
 
 
 def main():
@@ -134,16 +218,15 @@ def main():
     r1 = np.array([lib.genrand_real1() for _ in range(100)])
     written.append(save("mt19937", key=np.array([1234, 5678]), u32=u32, real1=r1))
 
-    manifest = {}
-    for p in written:
-        with open(p, "rb") as f:
-            manifest[os.path.basename(p)] = hashlib.sha256(f.read()).hexdigest()
-    with open(os.path.join(OUT, "MANIFEST.json"), "w") as f:
-        json.dump({"generator": "tests/golden/make_golden.py",
-                   "reference": "QuEST 3.4 CPU backend (Code/QuEST340/QuEST), QuEST_PREC=2, -fopenmp",
-                   "files": manifest}, f, indent=1, sort_keys=True)
+    written += main_f32()
+    write_manifest(written)
     print("\n".join(written))
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--f32"]:  # only the float fixtures
+        files = main_f32()
+        write_manifest(files)
+        print("\n".join(files))
+    else:
+        main()

@@ -434,3 +434,31 @@ def test_ops_oracle_gates_match_dense_operators(cpuref):
         O.run_ops(cpuref, q, [op])
         assert np.max(np.abs(cpuref.state(q) - M @ psi)) < 1e-14, op
         cpuref.destroy(q)
+
+
+def test_f32_golden_fixtures_agree_with_fp64_reference(reflib):
+    """The float fixtures (QuEST's CPU build at QuEST_PREC=1) against the same
+    inputs on its double build: equal to float resolution."""
+    g = golden("f32_brick12_extras")
+    n = int(g["n"])
+    q = reflib.qureg(n)
+    W.apply_gates(reflib, q, W.brick_circuit(n, int(g["layers"]), int(g["seed"]), extra_every=5))
+    assert np.max(np.abs(reflib.state(q) - g["state"])) < 1e-5
+    reflib.destroy(q)
+    g = golden("f32_density5_channels")
+    n = int(g["n"])
+    rho, pure = reflib.qureg(n, density=True), reflib.qureg(n)
+    channel_circuit(reflib, rho, pure, n, int(g["layers"]), int(g["seed"]))
+    assert np.max(np.abs(reflib.state(rho) - g["state"])) < 1e-5
+    assert abs(reflib.calcPurity(rho) - float(g["purity"])) < 1e-5
+    reflib.destroy(rho)
+    reflib.destroy(pure)
+    g = golden("f32_trotter8")
+    h = reflib.hamil(g["coeffs"], g["codes"])
+    for order in (1, 2, 4):
+        q = reflib.qureg(int(g["n"]))
+        reflib.set_state(q, random_state(int(g["n"]), seed=26))
+        reflib.applyTrotterCircuit(q, h, float(g["time"]), order, int(g["reps"]))
+        # order 4: 1800 float rotation sweeps in QuEST's float build drift to 1.2e-5
+        assert np.max(np.abs(reflib.state(q) - g[f"order{order}"])) < 3e-5
+        reflib.destroy(q)
