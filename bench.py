@@ -512,6 +512,50 @@ def cpu_swap_baseline(data, seconds_per_size=2.0):
             "max_p0_error_vs_closed_form": err}
 
 
+def cpu_swap_same_algorithm(data, seconds=10.0, seed=7):
+    """The same closed-form evaluation on the host cores: for each swap test,
+    all 4^q ancilla-0 amplitudes (a_x b_y + a_y b_x)/2 squared and summed in
+    FP64 (oracle/cpuref/swap_closed_form.c, OpenMP over tests), on the
+    normalised point and centroid registers. Sample: tests drawn uniformly from
+    the workload's 262144 (so q follows its distribution), in batches of 512
+    until `seconds` is spent."""
+    import ctypes as C
+
+    import numpy as np
+    from quantumlab_amd.quest import QuEST
+    lib = QuEST(os.path.join(ROOT, "oracle", "build", "libcpuref.so"))
+    fn = lib.lib.cpurefSwapP0ClosedForm
+    dp, lp = C.POINTER(C.c_double), C.POINTER(C.c_longlong)
+    fn.argtypes = [C.c_int, C.POINTER(C.c_int), dp, lp, dp, lp, dp]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    rng = np.random.default_rng(seed)
+    C_ = data.init.shape[0]
+    done, spent, err = 0, 0.0, 0.0
+    while spent < seconds:
+        picks = rng.integers(0, len(data.qubits) * C_, size=512)
+        qs = np.array([int(data.qubits[t // C_]) for t in picks], dtype=np.int32)
+        a_parts, b_parts = [], []
+        for t, q in zip(picks, qs):
+            a = data.point(int(t // C_))
+            b = data.init[int(t % C_), :1 << int(q)]
+            a_parts.append(a / np.linalg.norm(a))
+            nb = np.linalg.norm(b)
+            b_parts.append(b / nb if nb > 0 else b)
+        off = np.concatenate([[0], np.cumsum(1 << qs.astype(np.int64))[:-1]]).astype(np.int64)
+        A, B = np.concatenate(a_parts), np.concatenate(b_parts)
+        out = np.zeros(len(picks))
+        t0 = time.perf_counter()
+        fn(len(picks), qs.ctypes.data_as(C.POINTER(C.c_int)), A.ctypes.data_as(dp), off.ctypes.data_as(lp),
+           B.ctypes.data_as(dp), off.ctypes.data_as(lp), out.ctypes.data_as(dp))
+        spent += time.perf_counter() - t0
+        done += len(picks)
+        err = max(err, float(np.max(np.abs(out - 0.5 * (1 + np.array([a @ b for a, b in zip(a_parts, b_parts)]) ** 2)))))
+    return {"value": round(done / spent, 1), "unit": "swap-tests/s", "cores": threads, "kind": "port",
+            "sample": f"{done} swap tests drawn uniformly from the workload (q = 4..10), every ancilla-0 amplitude "
+                      f"evaluated in FP64 as the GPU plan does (C/OpenMP, {threads} threads); {spent:.2f} s",
+            "max_p0_error_vs_inner_product": err}
+
+
 def statevec_leg(qe, args, n, N):
     """The headline: per-gate FP64 brick circuit, value, roofline."""
     import numpy as np
@@ -593,7 +637,22 @@ def main():
     ap.add_argument("--swap-points", type=int, default=16384)
     ap.add_argument("--swap-centroids", type=int, default=16)
     ap.add_argument("--swap-iterations", type=int, default=10)
+    ap.add_argument("--only", choices=["statevec", "fused", "fused-large", "density", "trotter", "swap"],
+                    help="run one leg (for profiling it alone); 'fused' includes the FP64 per-gate run it is "
+                         "compared with, the others skip the headline leg; no CPU baselines")
     args = ap.parse_args()
+    if args.only:
+        args.no_cpu = True
+        args.no_fused = args.only not in ("fused", "fused-large")
+        args.no_density = args.only != "density"
+        args.no_trotter = args.only != "trotter"
+        args.no_swap = args.only != "swap"
+        if args.only == "fused-large":
+            args.fused_large_only = True
+        elif args.only not in ("statevec", "fused"):
+            args.steps = 0
+        if args.only == "statevec":
+            args.fp32_qubits = 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the state-vector "
               f"path here has no multi-GPU sharding; see DESIGN.md for how it would shard)", file=sys.stderr)
@@ -610,7 +669,7 @@ def main():
     n = args.qubits
     N = 1 << n
 
-    if args.steps > 0:
+    if args.steps > 0 and not getattr(args, "fused_large_only", False):
         result, q64, plans, timed, wall = statevec_leg(qe, args, n, N)
     else:
         # profiling runs of the other legs only: no headline value
@@ -618,9 +677,9 @@ def main():
         result = {"metric": METRIC, "value": None, "unit": "GB/s", "n_gpus": 1, "steps": 0,
                   "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
                   "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-                  "config": {"workload": "state-vector leg skipped (--steps 0)", "qubits": n}}
+                  "config": {"workload": "state-vector leg skipped (--steps 0 or --only)", "qubits": n}}
 
-    if not args.no_fused and q64 is not None:
+    if not args.no_fused and q64 is not None and not getattr(args, "fused_large_only", False):
         q32, wall32, blocks, norm2 = run_fp32_fused(qe, n, plans, args.warmup, args.steps, args.fuse_qubits)
         ip = qe.calcInnerProduct(q64, q32)
         # normalised fidelity: norms from FP64 reductions (the FP32-stored vector
@@ -645,7 +704,7 @@ def main():
         qe.destroy(q32)
     if q64 is not None:
         qe.destroy(q64)
-    if not args.no_fused and args.steps > 0 and args.fp32_qubits > 0:
+    if not args.no_fused and (args.steps > 0 or getattr(args, "fused_large_only", False)) and args.fp32_qubits > 0:
         result["fused_fp32_large"] = run_fp32_large(qe, args.fp32_qubits, args.seed, args.fuse_qubits)
 
     if not args.no_density:
@@ -678,8 +737,12 @@ def main():
             result["trotter"]["cpu_baseline"] = cpu_trotter_baseline(trotter_rows, args.trotter_qubits)
             log(f"cpu trotter baseline: {result['trotter']['cpu_baseline']['value']} ms per solve")
         if swap_data is not None:
-            result["swap_tests"]["cpu_baseline"] = cpu_swap_baseline(swap_data)
-            log(f"cpu swap-test baseline: {result['swap_tests']['cpu_baseline']['value']} swap-tests/s")
+            result["swap_tests"]["cpu_baseline"] = cpu_swap_same_algorithm(swap_data)
+            log(f"cpu swap-test baseline (same closed form): {result['swap_tests']['cpu_baseline']['value']} "
+                f"swap-tests/s")
+            result["swap_tests"]["cpu_reference_method"] = cpu_swap_baseline(swap_data)
+            log(f"cpu swap-test baseline (reference gate-level method): "
+                f"{result['swap_tests']['cpu_reference_method']['value']} swap-tests/s")
     print(json.dumps(result), flush=True)
 
 

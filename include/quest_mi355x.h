@@ -149,6 +149,11 @@ void qmxCircuitMultiQubitUnitary(qmxCircuit *c, int *targs, int numTargs, const 
 int qmxCircuitFuse(qmxCircuit *c, int maxQubits);
 /* dense blocks made by the last qmxCircuitFuse */
 int qmxCircuitNumBlocks(const qmxCircuit *c);
+/* Step `step` of the last qmxCircuitFuse (0 .. its return value - 1): for a
+ * block, its qubits (ascending; targs[0] = least significant bit of the
+ * matrix index) and its 2^k x 2^k matrix (row-major; any pointer may be NULL),
+ * returning k; 0 for a collapse; -1 past the end. */
+int qmxCircuitStep(const qmxCircuit *c, int step, int *targs, qreal *re, qreal *im);
 /* Apply to a register of the circuit's width; returns the product of the
  * collapse probabilities (1 without collapses). fused != 0 on a state vector
  * uses the fused blocks (after qmxCircuitFuse), otherwise each operation is the

@@ -300,7 +300,7 @@ double runOp(const Qureg &q, Op &o) {
         case OP_UNITARY: unitary(q, o.target, o.m2); break;
         case OP_ROTATE_Z: rotateZ(q, o.target, t); break;
         case OP_MULTI_CONTROLLED_UNITARY:
-            multiControlledUnitary(q, o.qubits.data(), (int)o.qubits.size() - 1, o.target, o.m2);
+            multiControlledUnitary(q, o.qubits.data(), (int)o.qubits.size(), o.target, o.m2);
             break;
         case OP_MULTI_QUBIT_UNITARY: multiQubitUnitary(q, o.qubits.data(), (int)o.qubits.size(), o.mN); break;
         default:
@@ -512,6 +512,7 @@ qmxCircuit *qmxCircuitCreate(int numQubits) {
     std::memset(&c->shape, 0, sizeof c->shape);
     c->shape.numQubitsRepresented = numQubits;
     c->shape.numQubitsInStateVec = numQubits;
+    c->shape.numAmpsPerChunk = c->shape.numAmpsTotal = numQubits < 63 ? 1LL << numQubits : 0;
     return c;
     CIRCUIT_CATCH(nullptr)
 }
@@ -634,6 +635,22 @@ int qmxCircuitFuse(qmxCircuit *c, int maxQubits) {
     CIRCUIT_CATCH(0)
 }
 
+int qmxCircuitStep(const qmxCircuit *c, int step, int *targs, qreal *re, qreal *im) {
+    if (step < 0 || step >= (int)c->blocks.size()) return -1;
+    const Block &b = c->blocks[step];
+    if (b.barrierOp >= 0) return 0;
+    const int k = (int)b.targs.size();
+    const size_t dim = 1ull << k;
+    if (targs)
+        for (int j = 0; j < k; ++j) targs[j] = b.targs[j];
+    if (re && im)
+        for (size_t e = 0; e < dim * dim; ++e) {
+            re[e] = (qreal)b.u[e].re;
+            im[e] = (qreal)b.u[e].im;
+        }
+    return k;
+}
+
 int qmxCircuitNumBlocks(const qmxCircuit *c) {
     int n = 0;
     for (const Block &b : c->blocks) n += b.barrierOp < 0;
@@ -655,6 +672,7 @@ qreal qmxApplyOperations(Qureg qureg, const qmxBasicOperation *ops, int numOps, 
     std::memset(&c.shape, 0, sizeof c.shape);
     c.shape.numQubitsRepresented = c.numQubits;
     c.shape.numQubitsInStateVec = c.numQubits;
+    c.shape.numAmpsPerChunk = c.shape.numAmpsTotal = c.numQubits < 63 ? 1LL << c.numQubits : 0;
     checkRegister(&c, qureg, __func__);
     if (numOps < 0) qv::raise("Invalid number of operations. Must be >=0.", __func__);
     if (noise) {

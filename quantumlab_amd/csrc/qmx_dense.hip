@@ -22,8 +22,7 @@ namespace {
 
 constexpr int kMaxB = 5;
 constexpr int kMaxEntries = 1 << (2 * kMaxB);   // 1024 complex entries
-constexpr int kRing = 256;
-constexpr size_t kSlotBytes = kMaxEntries * sizeof(Amp<double>);
+static_assert(kMaxEntries * sizeof(Amp<double>) <= kStageSlotBytes, "a dense matrix fits a staging slot");
 
 struct DenseSpec {
     int fix[kMaxB];                // ascending bit positions
@@ -398,33 +397,16 @@ __global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<T> *__restrict__ a,
     }
 }
 
-struct MatrixRing {
-    char *dev = nullptr, *host = nullptr;
-    hipEvent_t ev[kRing];
-    bool used[kRing] = {};
-    int next = 0;
-};
-MatrixRing g_ring;
-
+// matrices reach the device through the runtime's staging ring (no
+// allocation or synchronisation per launch)
 template <typename T>
 const Amp<T> *stageMatrix(const Cplx *M, int entries, hipStream_t st) {
-    if (!g_ring.dev) {
-        g_ring.dev = static_cast<char *>(devAlloc(kRing * kSlotBytes));
-        QMX_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&g_ring.host), kRing * kSlotBytes,
-                                    hipHostMallocDefault));
-        for (int i = 0; i < kRing; ++i)
-            QMX_HIP_CHECK(hipEventCreateWithFlags(&g_ring.ev[i], hipEventDisableTiming));
-    }
-    const int slot = g_ring.next;
-    g_ring.next = (slot + 1) % kRing;
-    if (g_ring.used[slot]) QMX_HIP_CHECK(hipEventSynchronize(g_ring.ev[slot]));
-    Amp<T> *h = reinterpret_cast<Amp<T> *>(g_ring.host + slot * kSlotBytes);
+    const size_t bytes = entries * sizeof(Amp<T>);
+    const Staged s = stageBegin(bytes);
+    Amp<T> *h = static_cast<Amp<T> *>(s.host);
     for (int e = 0; e < entries; ++e) h[e] = Amp<T>{(T)M[e].re, (T)M[e].im};
-    char *d = g_ring.dev + slot * kSlotBytes;
-    QMX_HIP_CHECK(hipMemcpyAsync(d, h, entries * sizeof(Amp<T>), hipMemcpyHostToDevice, st));
-    QMX_HIP_CHECK(hipEventRecord(g_ring.ev[slot], st));
-    g_ring.used[slot] = true;
-    return reinterpret_cast<const Amp<T> *>(d);
+    stageCommit(s, bytes, st);
+    return static_cast<const Amp<T> *>(s.dev);
 }
 
 template <typename T, int B>

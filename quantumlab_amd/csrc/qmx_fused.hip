@@ -128,19 +128,17 @@ struct Fused5Prepared {
     float *M;  // device copy of the 64x64 real block matrix
 };
 
-}  // namespace
-
-// Build the device-resident form of a 5-qubit block once (targs[0] = least
-// significant bit of u's index, u row-major 32x32).
-void *fused5Prepare(const int *targs, const Cplx *u) {
-    auto *p = new Fused5Prepared;
+// The kernel's form of a 5-qubit block (targs[0] = least significant bit of
+// u's index, u row-major 32x32): the spec, and the 64x64 real matrix
+// [[Ur, -Ui], [Ui, Ur]] written to hm.
+void fused5Build(const int *targs, const Cplx *u, Fused5Spec &spec, float *hm) {
     // index the block by ascending target: bit j of a combination is the j-th
     // lowest target, so a wave's 4 combinations (MFMA K lanes) run over the
     // lowest targets and its loads cover whole cache lines
     int order[5] = {0, 1, 2, 3, 4};
     std::sort(order, order + 5, [&](int x, int y) { return targs[x] < targs[y]; });
-    p->spec.targetsSorted.n = 5;
-    for (int j = 0; j < 5; ++j) p->spec.targetsSorted.pos[j] = targs[order[j]];
+    spec.targetsSorted.n = 5;
+    for (int j = 0; j < 5; ++j) spec.targetsSorted.pos[j] = targs[order[j]];
     int toOld[32];
     for (int c = 0; c < 32; ++c) {
         uint64_t o = 0;
@@ -150,10 +148,9 @@ void *fused5Prepare(const int *targs, const Cplx *u) {
                 o |= 1ull << targs[order[j]];
                 old |= 1 << order[j];
             }
-        p->spec.off[c] = o;
+        spec.off[c] = o;
         toOld[c] = old;
     }
-    std::vector<float> hm(64 * 64);
     for (int r = 0; r < 32; ++r)
         for (int c = 0; c < 32; ++c) {
             const Cplx z = u[toOld[r] * 32 + toOld[c]];
@@ -162,9 +159,41 @@ void *fused5Prepare(const int *targs, const Cplx *u) {
             hm[(32 + r) * 64 + c] = (float)z.im;
             hm[(32 + r) * 64 + 32 + c] = (float)z.re;
         }
+}
+
+void launchFused5(QuregDev *q, const Fused5Spec &spec, const float *M) {
+    const long long numChunks = (q->numAmps >> 5) / 16;
+    const long long wavesNeeded = (numChunks + kChunksPerWave - 1) / kChunksPerWave;
+    const long long grid = (wavesNeeded + (kBlock / 64) - 1) / (kBlock / 64);
+    hipLaunchKernelGGL(kFused5, dim3((unsigned)grid), dim3(kBlock), 0, hstream(), (Amp<float> *)q->amps,
+                       numChunks, spec, M);
+    checkLaunch("fused5Apply");
+}
+
+}  // namespace
+
+// Build the device-resident form of a 5-qubit block once, for blocks applied
+// many times (circuits).
+void *fused5Prepare(const int *targs, const Cplx *u) {
+    auto *p = new Fused5Prepared;
+    std::vector<float> hm(64 * 64);
+    fused5Build(targs, u, p->spec, hm.data());
     p->M = static_cast<float *>(devAlloc(hm.size() * sizeof(float)));
     hostToDevice(p->M, hm.data(), hm.size() * sizeof(float));
     return p;
+}
+
+// One application of a block (multiQubitUnitary on an FP32 register): the
+// matrix goes through the staging ring, with no allocation or synchronisation.
+void fused5ApplyOnce(QuregDev *q, const int *targs, const Cplx *u) {
+    constexpr size_t bytes = 64 * 64 * sizeof(float);
+    static_assert(bytes <= kStageSlotBytes, "a fused block fits a staging slot");
+    Fused5Spec spec;
+    hipStream_t st = hstream();
+    const Staged s = stageBegin(bytes);
+    fused5Build(targs, u, spec, static_cast<float *>(s.host));
+    stageCommit(s, bytes, st);
+    launchFused5(q, spec, static_cast<const float *>(s.dev));
 }
 
 void fused5Free(void *prepared) {
@@ -178,12 +207,7 @@ void fused5Free(void *prepared) {
 // chunk is 512 amplitudes).
 void fused5Apply(QuregDev *q, const void *prepared) {
     const auto *p = static_cast<const Fused5Prepared *>(prepared);
-    const long long numChunks = (q->numAmps >> 5) / 16;
-    const long long wavesNeeded = (numChunks + kChunksPerWave - 1) / kChunksPerWave;
-    const long long grid = (wavesNeeded + (kBlock / 64) - 1) / (kBlock / 64);
-    hipLaunchKernelGGL(kFused5, dim3((unsigned)grid), dim3(kBlock), 0, hstream(), (Amp<float> *)q->amps,
-                       numChunks, p->spec, (const float *)p->M);
-    checkLaunch("fused5Apply");
+    launchFused5(q, p->spec, p->M);
 }
 
 }  // namespace qmx

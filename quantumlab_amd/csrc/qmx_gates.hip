@@ -10,6 +10,7 @@
 // Consecutive lanes get consecutive w, so every load/store instruction of a
 // wave64 covers contiguous memory for all bits below the lowest fixed bit.
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "qmx_device.h"
@@ -333,10 +334,6 @@ BitList makeBits(uint64_t mask) {
 
 template <typename T> C2<T> c2(const Cplx &z) { return C2<T>{(T)z.re, (T)z.im}; }
 
-// device copy of a dense matrix, cached by content size (grown on demand)
-void *g_matBuf = nullptr;
-size_t g_matBytes = 0;
-
 int lowestBit(uint64_t m) { return m ? __builtin_ctzll(m) : 63; }
 
 // Every control among the lowest bits (below the nontemporal threshold): each
@@ -490,9 +487,7 @@ void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const
     const int n = q->numQubits;
     if (q->prec == FP32 && k == 5 && ctrlMask == 0 && n >= 9) {
         // uncontrolled 5-qubit matrices on FP32 registers run on the MFMA kernel
-        void *p = fused5Prepare(targs, u);
-        fused5Apply(q, p);
-        fused5Free(p);
+        fused5ApplyOnce(q, targs, u);
         return;
     }
     if (ctrlMask == 0 && denseBitsSupported(k) && n >= k) {
@@ -540,20 +535,22 @@ void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const
         size_t matBytes = sizeof(C2<T>) * (size_t)dim * dim;
         std::vector<C2<T>> hm((size_t)dim * dim);
         for (size_t e = 0; e < hm.size(); ++e) hm[e] = c2<T>(u[e]);
-        if (g_matBytes < matBytes) {
-            devFree(g_matBuf);
-            g_matBuf = devAlloc(matBytes);
-            g_matBytes = matBytes;
+        hipStream_t st = hstream();
+        const void *dMat;
+        if (matBytes <= kStageSlotBytes) {
+            const Staged sg = stageBegin(matBytes);
+            std::memcpy(sg.host, hm.data(), matBytes);
+            stageCommit(sg, matBytes, st);
+            dMat = sg.dev;
+        } else {
+            dMat = stageLarge(hm.data(), matBytes, st);
         }
-        QMX_HIP_CHECK(hipMemcpyAsync(g_matBuf, hm.data(), matBytes, hipMemcpyHostToDevice, hstream()));
         size_t tileBytes = sizeof(Amp<T>) * tileAmps;
         int matInLds = (tileBytes + matBytes + dim * sizeof(int) <= 64 * 1024) ? 1 : 0;
         size_t shm = tileBytes + (matInLds ? matBytes : 0) + dim * sizeof(int);
         long long grid = std::min<long long>(numTiles, 65536);
-        hipLaunchKernelGGL((kMatrixNTile<T, 16>), dim3((unsigned)grid), dim3(kBlock), shm, hstream(),
-                           (Amp<T> *)q->amps, numTiles, ts, (const C2<T> *)g_matBuf, matInLds);
-        // the host vector must outlive the async copy
-        QMX_HIP_CHECK(hipStreamSynchronize(hstream()));
+        hipLaunchKernelGGL((kMatrixNTile<T, 16>), dim3((unsigned)grid), dim3(kBlock), shm, st,
+                           (Amp<T> *)q->amps, numTiles, ts, (const C2<T> *)dMat, matInLds);
     });
     checkLaunch("applyMatrixN");
 }

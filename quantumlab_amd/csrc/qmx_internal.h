@@ -53,6 +53,9 @@ void *stream();                     // hipStream_t of the library
 void deviceSync();
 void *devAlloc(size_t bytes);       // fails loudly
 void devFree(void *p);
+// register buffers: small ones recycled through per-size free lists (nullptr on failure)
+void *poolAlloc(size_t bytes);
+void poolFree(void *p, size_t bytes);
 void hostToDevice(void *dst, const void *src, size_t bytes);
 void deviceToHost(void *dst, const void *src, size_t bytes);
 double *pinnedScratch();            // 64 doubles of pinned host memory
@@ -64,6 +67,20 @@ double timerEndMs();
 void setPendingWork(void (*fn)());
 int deviceDescription(char *buf, int len);
 int numCUs();                       // compute units of the device
+// Staging ring for small per-launch inputs: fill Staged::host (bytes <=
+// kStageSlotBytes), then stageCommit queues its copy on `stream`; Staged::dev
+// stays valid for the next kernels on that stream. No allocation, and the host
+// waits only when a slot comes round again before its previous copy finished.
+constexpr size_t kStageSlotBytes = 16384;
+struct Staged {
+    void *host;
+    void *dev;
+    int slot;
+};
+Staged stageBegin(size_t bytes);
+void stageCommit(const Staged &s, size_t bytes, void *stream);
+// an input of any size through one growable pinned / device pair
+const void *stageLarge(const void *src, size_t bytes, void *stream);
 // raise a kernel's dynamic-LDS limit to `bytes` (beyond the 64 KiB default);
 // false, with a message, when the device has less
 bool allowDynamicLds(const void *kernel, size_t bytes, const char *caller);
@@ -103,6 +120,8 @@ void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const
 void *fused5Prepare(const int *targs, const Cplx *u);
 void fused5Free(void *prepared);
 void fused5Apply(QuregDev *q, const void *prepared);
+// one application without a prepared form (staged per call, no allocation)
+void fused5ApplyOnce(QuregDev *q, const int *targs, const Cplx *u);
 // exp(-i angle/2 * P), P = X^{xMask} Z^{zMask} with Y = iXZ folded in (single sweep)
 void applyPauliRotation(QuregDev *q, uint64_t ctrlMask, uint64_t xMask, uint64_t zMask,
                         int numY, double angle);
@@ -143,7 +162,9 @@ double densFidelity(const QuregDev *rho, const QuregDev *psi);     // <psi|rho|p
 void probOfAllOutcomes(const QuregDev *q, const int *qubits, int k, double *out);
 // device doubles probOfAllOutcomesInto needs as scratch for these qubits
 long long outcomeScratchDoubles(const QuregDev *q, const int *qubits, int k);
-// the same histogram with a caller-provided device scratch (outcomeScratchDoubles)
+// the same histogram with a caller-provided device scratch (outcomeScratchDoubles):
+// results left on the device (the first 2^k doubles of scratch), or copied to out
+const double *probOfAllOutcomesDevice(const QuregDev *q, const int *qubits, int k, double *scratch);
 void probOfAllOutcomesInto(const QuregDev *q, const int *qubits, int k, double *scratch, double *out);
 double normSquaredThenScale(QuregDev *q, bool rescale);            // FP64 norm, optional 1/sqrt renorm
 double maxAbsDiff(const QuregDev *a, const QuregDev *b);           // max_i |a_i - b_i|

@@ -400,7 +400,7 @@ long long outcomeScratchDoubles(const QuregDev *q, const int *qubits, int k) {
     return pl.chunkLog ? (numOut << pl.chunkLog) + numOut : numOut;
 }
 
-void probOfAllOutcomesInto(const QuregDev *q, const int *qubits, int k, double *scratch, double *out) {
+const double *probOfAllOutcomesDevice(const QuregDev *q, const int *qubits, int k, double *scratch) {
     const OutcomePlan pl = outcomePlan(q, qubits, k);
     const bool dens = q->isDensity != 0;
     const long long dim = 1ll << q->numQubitsRep;
@@ -423,7 +423,12 @@ void probOfAllOutcomesInto(const QuregDev *q, const int *qubits, int k, double *
                            hstream(), part, pl.chunkLog, numOut, res);
         checkLaunch("probOfAllOutcomes finish");
     }
-    QMX_HIP_CHECK(hipMemcpyAsync(out, res, numOut * sizeof(double), hipMemcpyDeviceToHost, hstream()));
+    return res;
+}
+
+void probOfAllOutcomesInto(const QuregDev *q, const int *qubits, int k, double *scratch, double *out) {
+    const double *res = probOfAllOutcomesDevice(q, qubits, k, scratch);
+    QMX_HIP_CHECK(hipMemcpyAsync(out, res, ((size_t)1 << k) * sizeof(double), hipMemcpyDeviceToHost, hstream()));
     QMX_HIP_CHECK(hipStreamSynchronize(hstream()));
 }
 

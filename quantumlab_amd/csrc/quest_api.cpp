@@ -342,7 +342,7 @@ Qureg makeQureg(int numQubitsInSV, int isDensity, int numRep, int prec, const ch
         qv::check(false, qv::E_QUREG_NOT_ALLOCATED, caller);
     }
     size_t bytes = (size_t)numAmps * qmx::ampBytes(prec);
-    d->amps = qmxTryDevAlloc(bytes);
+    d->amps = qmx::poolAlloc(bytes);
     if (!d->amps) {
         munmap(re, d->hostBytes);
         munmap(im, d->hostBytes);
@@ -587,7 +587,7 @@ void destroyQureg(Qureg qureg, QuESTEnv env) {
     QuregDev *d = D(qureg);
     if (!d || d->magic != qmx::kMagic) return;
     qmx::densDropPending(d);
-    qmx::devFree(d->amps);
+    qmx::poolFree(d->amps, (size_t)d->numAmps * qmx::ampBytes(d->prec));
     munmap(qureg.stateVec.real, d->hostBytes);
     munmap(qureg.stateVec.imag, d->hostBytes);
     std::free(qureg.qasmLog);
@@ -1968,14 +1968,16 @@ Complex calcExpecDiagonalOp(Qureg qureg, DiagonalOp op) {
 /* ---- phase functions and QFT (QuEST.c:732-891) -------------------------------------------------- */
 
 // each public entry validates under its own name (the reference's __func__)
+// withOverrides: called from an *Overrides entry point, whose override count
+// is validated even when the index array is NULL (QuEST.c:732-891)
 static void phaseFuncImpl(const Qureg &qureg, int *qubits, int numQubits, int encoding, qreal *coeffs,
                           qreal *exponents, int numTerms, long long *overrideInds, qreal *overridePhases,
-                          int numOverrides, const char *caller) {
+                          int numOverrides, bool withOverrides, const char *caller) {
     const Doubles c(coeffs, (size_t)std::max(0, numTerms)), e(exponents, (size_t)std::max(0, numTerms));
     const Doubles ph(overridePhases, (size_t)std::max(0, numOverrides));
     qv::phaseMultiQubits(qureg, qubits, numQubits, caller);
     qv::bitEncoding(numQubits, encoding, caller);
-    if (overrideInds) qv::phaseFuncOverrides(numQubits, encoding, overrideInds, numOverrides, caller);
+    if (withOverrides) qv::phaseFuncOverrides(numQubits, encoding, overrideInds, numOverrides, caller);
     qv::phaseFuncTerms(numQubits, encoding, e.get(), numTerms, overrideInds, numOverrides, caller);
     qmx::PhaseFuncArgs a{};
     a.numRegs = 1;
@@ -1988,13 +1990,13 @@ static void phaseFuncImpl(const Qureg &qureg, int *qubits, int numQubits, int en
     a.numTermsPerReg = &numTerms;
     a.overrideInds = overrideInds;
     a.overridePhases = ph.get();
-    a.numOverrides = overrideInds ? numOverrides : 0;
+    a.numOverrides = withOverrides && overrideInds ? numOverrides : 0;
     phaseFuncPasses(qureg, a);
 }
 
 static void multiVarImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, int numRegs, int encoding,
                          qreal *coeffs, qreal *exponents, int *numTermsPerReg, long long *overrideInds,
-                         qreal *overridePhases, int numOverrides, const char *caller) {
+                         qreal *overridePhases, int numOverrides, bool withOverrides, const char *caller) {
     qv::qubitSubregs(qureg, qubits, numQubitsPerReg, numRegs, caller);
     size_t totalTerms = 0;
     for (int r = 0; r < numRegs; ++r) totalTerms += (size_t)std::max(0, numTermsPerReg[r]);
@@ -2002,7 +2004,7 @@ static void multiVarImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, 
     const Doubles ph(overridePhases, (size_t)std::max(0, numOverrides));
     qv::multiRegBitEncoding(numQubitsPerReg, numRegs, encoding, caller);
     qv::multiVarPhaseFuncTerms(numRegs, encoding, e.get(), numTermsPerReg, caller);
-    if (overrideInds)
+    if (withOverrides)
         qv::multiVarPhaseFuncOverrides(numQubitsPerReg, numRegs, encoding, overrideInds, numOverrides, caller);
     qmx::PhaseFuncArgs a{};
     a.numRegs = numRegs;
@@ -2015,19 +2017,19 @@ static void multiVarImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, 
     a.numTermsPerReg = numTermsPerReg;
     a.overrideInds = overrideInds;
     a.overridePhases = ph.get();
-    a.numOverrides = overrideInds ? numOverrides : 0;
+    a.numOverrides = withOverrides && overrideInds ? numOverrides : 0;
     phaseFuncPasses(qureg, a);
 }
 
 static void namedImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, int numRegs, int encoding, int code,
                       qreal *params, int numParams, long long *overrideInds, qreal *overridePhases, int numOverrides,
-                      const char *caller) {
+                      bool withOverrides, const char *caller) {
     const Doubles pr(params, (size_t)std::max(0, numParams));
     const Doubles ph(overridePhases, (size_t)std::max(0, numOverrides));
     qv::qubitSubregs(qureg, qubits, numQubitsPerReg, numRegs, caller);
     qv::multiRegBitEncoding(numQubitsPerReg, numRegs, encoding, caller);
     qv::phaseFuncName(code, numRegs, numParams, caller);
-    if (overrideInds)
+    if (withOverrides)
         qv::multiVarPhaseFuncOverrides(numQubitsPerReg, numRegs, encoding, overrideInds, numOverrides, caller);
     qmx::PhaseFuncArgs a{};
     a.numRegs = numRegs;
@@ -2040,14 +2042,14 @@ static void namedImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, int
     a.numParams = params ? numParams : 0;
     a.overrideInds = overrideInds;
     a.overridePhases = ph.get();
-    a.numOverrides = overrideInds ? numOverrides : 0;
+    a.numOverrides = withOverrides && overrideInds ? numOverrides : 0;
     phaseFuncPasses(qureg, a);
 }
 
 void applyPhaseFunc(Qureg qureg, int *qubits, int numQubits, enum bitEncoding encoding, qreal *coeffs,
                     qreal *exponents, int numTerms) {
     API_TRY
-    phaseFuncImpl(qureg, qubits, numQubits, encoding, coeffs, exponents, numTerms, nullptr, nullptr, 0, __func__);
+    phaseFuncImpl(qureg, qubits, numQubits, encoding, coeffs, exponents, numTerms, nullptr, nullptr, 0, false, __func__);
     API_CATCH_VOID
 }
 
@@ -2056,7 +2058,7 @@ void applyPhaseFuncOverrides(Qureg qureg, int *qubits, int numQubits, enum bitEn
                              int numOverrides) {
     API_TRY
     phaseFuncImpl(qureg, qubits, numQubits, encoding, coeffs, exponents, numTerms, overrideInds, overridePhases,
-                  numOverrides, __func__);
+                  numOverrides, true, __func__);
     API_CATCH_VOID
 }
 
@@ -2064,7 +2066,7 @@ void applyMultiVarPhaseFunc(Qureg qureg, int *qubits, int *numQubitsPerReg, int 
                             qreal *coeffs, qreal *exponents, int *numTermsPerReg) {
     API_TRY
     multiVarImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, coeffs, exponents, numTermsPerReg, nullptr,
-                 nullptr, 0, __func__);
+                 nullptr, 0, false, __func__);
     API_CATCH_VOID
 }
 
@@ -2073,7 +2075,7 @@ void applyMultiVarPhaseFuncOverrides(Qureg qureg, int *qubits, int *numQubitsPer
                                      long long int *overrideInds, qreal *overridePhases, int numOverrides) {
     API_TRY
     multiVarImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, coeffs, exponents, numTermsPerReg, overrideInds,
-                 overridePhases, numOverrides, __func__);
+                 overridePhases, numOverrides, true, __func__);
     API_CATCH_VOID
 }
 
@@ -2081,7 +2083,7 @@ void applyNamedPhaseFunc(Qureg qureg, int *qubits, int *numQubitsPerReg, int num
                          enum phaseFunc functionNameCode) {
     API_TRY
     namedImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, functionNameCode, nullptr, 0, nullptr, nullptr, 0,
-              __func__);
+              false, __func__);
     API_CATCH_VOID
 }
 
@@ -2090,7 +2092,7 @@ void applyNamedPhaseFuncOverrides(Qureg qureg, int *qubits, int *numQubitsPerReg
                                   long long int *overrideInds, qreal *overridePhases, int numOverrides) {
     API_TRY
     namedImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, functionNameCode, nullptr, 0, overrideInds,
-              overridePhases, numOverrides, __func__);
+              overridePhases, numOverrides, true, __func__);
     API_CATCH_VOID
 }
 
@@ -2098,7 +2100,7 @@ void applyParamNamedPhaseFunc(Qureg qureg, int *qubits, int *numQubitsPerReg, in
                               enum phaseFunc functionNameCode, qreal *params, int numParams) {
     API_TRY
     namedImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, functionNameCode, params, numParams, nullptr, nullptr,
-              0, __func__);
+              0, false, __func__);
     API_CATCH_VOID
 }
 
@@ -2108,7 +2110,7 @@ void applyParamNamedPhaseFuncOverrides(Qureg qureg, int *qubits, int *numQubitsP
                                        int numOverrides) {
     API_TRY
     namedImpl(qureg, qubits, numQubitsPerReg, numRegs, encoding, functionNameCode, params, numParams, overrideInds,
-              overridePhases, numOverrides, __func__);
+              overridePhases, numOverrides, true, __func__);
     API_CATCH_VOID
 }
 

@@ -441,7 +441,7 @@ class QuEST:
 
     # -- registers ---------------------------------------------------------
     def qureg(self, n: int, density: bool = False, precision: Optional[int] = None) -> Qureg:
-        if precision is not None and precision != QMX_PREC_FP64:
+        if precision is not None:
             q = (self.lib.createDensityQuregEx if density else self.lib.createQuregEx)(n, self.env, precision)
         else:
             q = (self.lib.createDensityQureg if density else self.lib.createQureg)(n, self.env)

@@ -182,3 +182,34 @@ def test_autotune_keeps_results(gpu):
     plan.run(1, graph=True)
     assert np.max(np.abs(plan.p0() - before)) < 1e-6
     plan.close()
+
+
+def test_closed_form_cpu_port_matches_inner_products(cpuref):
+    """The like-for-like CPU baseline (oracle/cpuref/swap_closed_form.c: every
+    ancilla-0 amplitude squared and summed) equals (1 + <a|b>^2)/2 for
+    normalised real registers of every size in the workload, and sums the
+    unnormalised form too (|a|^2 |b|^2 + <a|b>^2)/2."""
+    import ctypes as C
+    rng = np.random.default_rng(17)
+    qs = [1, 2, 4, 5, 7, 10, 3]
+    a_parts, b_parts, want = [], [], []
+    for q in qs:
+        a = rng.standard_normal(1 << q)
+        b = rng.standard_normal(1 << q)
+        if q != 3:
+            a /= np.linalg.norm(a)
+            b /= np.linalg.norm(b)
+        a_parts.append(a)
+        b_parts.append(b)
+        want.append((np.dot(a, a) * np.dot(b, b) + np.dot(a, b) ** 2) / 2)
+    off = np.concatenate([[0], np.cumsum([1 << q for q in qs])[:-1]]).astype(np.int64)
+    A, B = np.concatenate(a_parts), np.concatenate(b_parts)
+    out = np.zeros(len(qs))
+    fn = cpuref.lib.cpurefSwapP0ClosedForm
+    dp = C.POINTER(C.c_double)
+    lp = C.POINTER(C.c_longlong)
+    fn.argtypes = [C.c_int, C.POINTER(C.c_int), dp, lp, dp, lp, dp]
+    qa = np.array(qs, dtype=np.int32)
+    fn(len(qs), qa.ctypes.data_as(C.POINTER(C.c_int)), A.ctypes.data_as(dp), off.ctypes.data_as(lp),
+       B.ctypes.data_as(dp), off.ctypes.data_as(lp), out.ctypes.data_as(dp))
+    assert np.max(np.abs(out - np.array(want))) < 1e-13
