@@ -170,8 +170,36 @@ def run_fp32_large(qe, n, seed, k):
                          "mfma_tflops": round(blocks * N * 256 / wall / 1e12, 2)}}
 
 
+def host_cpu():
+    """The host the CPU baselines run on: model, physical cores, logical CPUs,
+    the CPUs this process may use, and the OpenMP settings in force."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None and core is not None:
+                cores.add((phys, core))
+                phys = core = None
+    except OSError:
+        pass
+    return {"model": model, "physical_cores": len(cores) or None, "logical_cpus": os.cpu_count(),
+            "cpus_allowed": len(os.sched_getaffinity(0)), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES")}
+
+
 def cpu_baseline(n_cpu, seed):
-    """QuEST's own CPU backend (oracle/_ref) on one 5-layer period at n_cpu qubits."""
+    """QuEST's own CPU backend (oracle/_ref) on one 5-layer period at n_cpu
+    qubits (30 by default: a 32-qubit period would take minutes; the per-gate
+    cost of these bandwidth-bound sweeps scales with the amplitude count, so
+    the 32-qubit figure is the 30-qubit time x 4)."""
     from quantumlab_amd import workloads as W
     from quantumlab_amd.quest import QuEST
     ref = os.path.join(ROOT, "oracle", "_ref", "libquest_ref.so")
@@ -190,7 +218,46 @@ def cpu_baseline(n_cpu, seed):
             "sample": f"{n_cpu}-qubit FP64 brick period ({PERIOD} layers, {len(gates)} gates, same generator) "
                       f"through QuEST's C API; {'QuEST 3.4 CPU backend compiled from the reference sources' if kind == 'reference' else 'C restatement of it'}, "
                       f"OpenMP {threads} threads; {dt:.2f} s",
-            "gates_per_s": round(len(gates) / dt, 2)}
+            "gates_per_s": round(len(gates) / dt, 2),
+            "s_per_period_scaled_to_32q": round(dt * (1 << (32 - n_cpu)), 2) if n_cpu <= 32 else None,
+            "host": host_cpu()}
+
+
+def run_stream(qe, n, reps=3):
+    """The achievable HBM rate on this box, from the library's own streaming
+    kernels on an n-qubit FP64 register (2^n x 16 B): calcTotalProb (kSumSq,
+    one read sweep), qmxRenormalise (kSumSq then kScale, an in-place read +
+    write sweep: kScale's time is the difference) and cloneQureg (the runtime's
+    device-to-device copy into a second register). Best of `reps`, HIP events on
+    the library stream. kScale is also the calibration kernel of the PMC
+    traffic summaries (it reads and writes every amplitude exactly once)."""
+    N = 1 << n
+    q, q2 = qe.qureg(n), qe.qureg(n)
+    qe.initPlusState(q)
+    best = {"read": 1e30, "renorm": 1e30, "copy": 1e30}
+    for _ in range(reps):
+        qe.qmxSync()
+        qe.qmxTimerBegin()
+        qe.calcTotalProb(q)
+        best["read"] = min(best["read"], qe.qmxTimerEndMs())
+        qe.qmxTimerBegin()
+        qe.qmxRenormalise(q, 1)
+        best["renorm"] = min(best["renorm"], qe.qmxTimerEndMs())
+        qe.qmxTimerBegin()
+        qe.cloneQureg(q2, q)
+        best["copy"] = min(best["copy"], qe.qmxTimerEndMs())
+    qe.destroy(q)
+    qe.destroy(q2)
+    nbytes = N * 16.0
+    scale_ms = best["renorm"] - best["read"]
+    res = {"qubits": n, "bytes_per_register": nbytes,
+           "read_sweep_ms": round(best["read"], 3), "read_GBps": round(nbytes / best["read"] / 1e6, 1),
+           "in_place_rw_ms": round(scale_ms, 3), "in_place_rw_GBps": round(2 * nbytes / scale_ms / 1e6, 1),
+           "copy_ms": round(best["copy"], 3), "copy_GBps": round(2 * nbytes / best["copy"] / 1e6, 1),
+           "peak_GBps": HBM_PEAK_GBS}
+    log(f"stream at {n} qubits: read {res['read_GBps']} GB/s, in-place r+w {res['in_place_rw_GBps']} GB/s, "
+        f"copy {res['copy_GBps']} GB/s")
+    return res
 
 
 def run_density(qe, n, layers, seed):
@@ -623,7 +690,7 @@ def main():
     ap.add_argument("--fuse-qubits", type=int, default=5)
     ap.add_argument("--fp32-qubits", type=int, default=33,
                     help="register size of the large FP32 fused run (0: skip)")
-    ap.add_argument("--cpu-qubits", type=int, default=27)
+    ap.add_argument("--cpu-qubits", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fused", action="store_true")
     ap.add_argument("--no-swap", action="store_true")
@@ -637,7 +704,7 @@ def main():
     ap.add_argument("--swap-points", type=int, default=16384)
     ap.add_argument("--swap-centroids", type=int, default=16)
     ap.add_argument("--swap-iterations", type=int, default=10)
-    ap.add_argument("--only", choices=["statevec", "fused", "fused-large", "density", "trotter", "swap"],
+    ap.add_argument("--only", choices=["statevec", "fused", "fused-large", "density", "trotter", "swap", "stream"],
                     help="run one leg (for profiling it alone); 'fused' includes the FP64 per-gate run it is "
                          "compared with, the others skip the headline leg; no CPU baselines")
     args = ap.parse_args()
@@ -657,6 +724,11 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the state-vector "
               f"path here has no multi-GPU sharding; see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+
+    if not args.no_cpu:
+        # OpenMP reads these when its runtime loads (with the first CPU library)
+        os.environ.setdefault("OMP_PROC_BIND", "close")
+        os.environ.setdefault("OMP_PLACES", "cores")
 
     import numpy as np
     import quantumlab_amd
@@ -706,6 +778,9 @@ def main():
         qe.destroy(q64)
     if not args.no_fused and (args.steps > 0 or getattr(args, "fused_large_only", False)) and args.fp32_qubits > 0:
         result["fused_fp32_large"] = run_fp32_large(qe, args.fp32_qubits, args.seed, args.fuse_qubits)
+
+    if args.only == "stream":
+        result["stream"] = run_stream(qe, n)
 
     if not args.no_density:
         result["density"] = run_density(qe, args.density_qubits, args.density_layers, args.seed)

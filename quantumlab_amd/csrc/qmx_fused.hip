@@ -1,19 +1,26 @@
-// Fused dense 5-qubit blocks on FP32 amplitudes, as a GEMM on MFMA.
+// Fused dense 5-qubit blocks on FP32 amplitudes, as GEMMs on MFMA.
 //
 // A block U (32x32 complex) acting on 5 target qubits maps every group of 32
 // amplitudes (one per target combination, the other qubits fixed) to U times
-// that group. Written on real and imaginary parts, a column x = [Re; Im] of
-// 64 floats goes to M x with M = [[Ur, -Ui], [Ui, Ur]] (64x64). For 16 groups
-// at once that is OUT(64x16) = M(64x64) * X(64x16): 4 output tiles of 16x16,
-// each the sum of 16 v_mfma_f32_16x16x4f32 steps over K = 64.
+// that group. For 16 groups at once, with X = Xr + i Xi (32x16), the product
+// is formed from three real 32x32 by 32x16 products (Gauss):
+//   P1 = Ur Xr,  P2 = Ui Xi,  P3 = (Ur + Ui)(Xr + Xi),
+//   Re(UX) = P1 - P2,  Im(UX) = P3 - P1 - P2,
+// each as 2 row tiles x 8 steps of v_mfma_f32_16x16x4f32: 48 MFMAs per 16
+// groups, where the real 64x64 form [[Ur, -Ui], [Ui, Ur]] takes 64. Ur + Ui is
+// formed on the host in FP64 and rounded once; Xr + Xi costs 8 FP32 adds per
+// lane. A row tile's real and imaginary outputs come from the same three
+// products, so a tile is finished and stored before the next one starts.
 //
 // Operand placement (16x16x4 f32 MFMA, wave64):
-//   A (16x4): lane l holds A[l%16][l/16]  -> M fragments, staged once per
-//             workgroup in LDS as [row tile][row][k lane][step] so a lane
-//             reads its 16 values of a row tile as four 16 B vectors;
+//   A (16x4): lane l holds A[l%16][l/16]  -> matrix fragments, staged once per
+//             workgroup in LDS as [product][row tile][row][k lane][step] with
+//             rows 36 floats apart (so 8 consecutive lanes cover the 32 banks),
+//             a lane reading its 8 values of a product and row tile as two
+//             16 B vectors;
 //   B (4x16): lane l holds B[l/16][l%16]  -> lane l loads amplitude
 //             (group l%16, combination 4s + l/16) for s = 0..7: its .x feeds
-//             K-slice s (real rows), its .y K-slice s+8 (imaginary rows);
+//             P1's step s, its .y P2's, their sum P3's;
 //   D (16x16): lane l holds D[4(l/16)+v][l%16], v = 0..3.
 // Amplitudes go from HBM straight into MFMA operands and back. The block is
 // indexed by ascending target (combination bit j = j-th lowest target), so a
@@ -22,9 +29,9 @@
 // groups. Each wave takes 4 consecutive 16-group chunks and loads the next
 // chunk's amplitudes before multiplying the current one.
 //
-// Cost per amplitude: 256 FP32 MFMA flops for 16 B of HBM traffic, which on
-// MI355X (157.3 TFLOP/s FP32 matrix, 8 TB/s HBM) keeps the block near the
-// memory bound; every fused block replaces ~10 per-gate sweeps.
+// Cost per amplitude: 192 FP32 MFMA flops for 16 B of HBM traffic (the 4-product
+// form: 256), which keeps the matrix cores well inside the time MI355X's HBM
+// needs for the sweep; every fused block replaces ~10 per-gate sweeps.
 #include <algorithm>
 #include <vector>
 
@@ -51,17 +58,22 @@ __device__ __forceinline__ uint64_t fusedGroupBase(uint64_t g, const BitList &t)
     return g;
 }
 
+// matrix fragments in LDS: [product][row tile][row: kRowStride floats][k lane][step]
+constexpr int kRowStride = 36;
+constexpr int kTileStride = 16 * kRowStride;
+constexpr int kProductStride = 2 * kTileStride;
+
 __global__ void __launch_bounds__(kBlock, 5) kFused5(Amp<float> *__restrict__ a, long long numChunks,
                                                   Fused5Spec spec, const float *__restrict__ M) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 15, kq = lane >> 4;
-    // A fragments staged in LDS as [rt][col][kq][s] so a lane's 16 values of a
-    // row tile are contiguous
-    __shared__ float sM[4 * 16 * 4 * 16];
-    for (int e = threadIdx.x; e < 4096; e += kBlock) {
-        const int rt = e >> 10, cc = (e >> 6) & 15, kk = (e >> 4) & 3, ss = e & 15;
-        sM[e] = M[(16 * rt + cc) * 64 + 4 * ss + kk];
+    __shared__ __attribute__((aligned(16))) float sM[3 * kProductStride];
+    for (int e = threadIdx.x; e < 3 * 1024; e += kBlock) {
+        // e = ((p * 2 + rt) * 16 + row) * 32 + kk * 8 + ss: M[p][16 rt + row][4 ss + kk]
+        const int p = e >> 10, rt = (e >> 9) & 1, row = (e >> 5) & 15, kk = (e >> 3) & 3, ss = e & 7;
+        sM[p * kProductStride + rt * kTileStride + row * kRowStride + kk * 8 + ss] =
+            M[p * 1024 + (16 * rt + row) * 32 + 4 * ss + kk];
     }
     __syncthreads();
 
@@ -74,6 +86,7 @@ __global__ void __launch_bounds__(kBlock, 5) kFused5(Amp<float> *__restrict__ a,
     // and waiting for them would wait for the prefetch too.
     const uint64_t inLane = kq == 0 ? 0 : kq == 1 ? spec.off[1] : kq == 2 ? spec.off[2] : spec.off[3];
     const uint64_t outLane = kq == 0 ? 0 : kq == 1 ? spec.off[4] : kq == 2 ? spec.off[8] : spec.off[12];
+    const float *lm = sM + col * kRowStride + kq * 8;
     // the next chunk's amplitudes are loaded while the current chunk multiplies
     Amp<float> x[8], xn[8];
     uint64_t base = 0, nbase = 0;
@@ -93,44 +106,47 @@ __global__ void __launch_bounds__(kBlock, 5) kFused5(Amp<float> *__restrict__ a,
 #pragma unroll
             for (int s = 0; s < 8; ++s) xn[s] = ldA<float, false>(a + ((nbase | inLane) + spec.off[4 * s]));
         }
-        f4 acc[4];
+        float xs[8];
 #pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
-            float Af[1][16];
-            const f4 *src = reinterpret_cast<const f4 *>(sM + (((rt * 16 + col) * 4 + kq) << 4));
+        for (int s = 0; s < 8; ++s) xs[s] = x[s].x + x[s].y;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const f4 v = src[u];
-                Af[0][4 * u] = v[0]; Af[0][4 * u + 1] = v[1]; Af[0][4 * u + 2] = v[2]; Af[0][4 * u + 3] = v[3];
+        for (int h = 0; h < 2; ++h) {
+            f4 p1 = f4{0.f, 0.f, 0.f, 0.f}, p2 = p1, p3 = p1;
+            const f4 *ar = reinterpret_cast<const f4 *>(lm + h * kTileStride);
+            const f4 *ai = reinterpret_cast<const f4 *>(lm + kProductStride + h * kTileStride);
+            const f4 *as = reinterpret_cast<const f4 *>(lm + 2 * kProductStride + h * kTileStride);
+            const f4 r0 = ar[0], r1 = ar[1], i0 = ai[0], i1 = ai[1], s0 = as[0], s1 = as[1];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r0[s], x[s].x, p1, 0, 0, 0);
+                p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(i0[s], x[s].y, p2, 0, 0, 0);
+                p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(s0[s], xs[s], p3, 0, 0, 0);
             }
-            acc[rt] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int s = 0; s < 8; ++s)
-                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[0][s], x[s].x, acc[rt], 0, 0, 0);
+            for (int s = 0; s < 4; ++s) {
+                p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r1[s], x[s + 4].x, p1, 0, 0, 0);
+                p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(i1[s], x[s + 4].y, p2, 0, 0, 0);
+                p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(s1[s], xs[s + 4], p3, 0, 0, 0);
+            }
+            // lane holds output rows 4 kq + v of tile h: combination c = 16 h + 4 kq + v
 #pragma unroll
-            for (int s = 0; s < 8; ++s)
-                acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(Af[0][s + 8], x[s].y, acc[rt], 0, 0, 0);
-        }
-        // lane holds output rows 4 kq + v of each tile: combination c = 16 h + 4 kq + v,
-        // real part from tile h, imaginary part from tile h + 2
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
+            for (int v = 0; v < 4; ++v)
                 stA<float, false>(a + ((base | outLane) + spec.off[16 * h + v]),
-                                  Amp<float>{acc[h][v], acc[h + 2][v]});
-            }
+                                  Amp<float>{p1[v] - p2[v], p3[v] - p1[v] - p2[v]});
+        }
     }
 }
 
 struct Fused5Prepared {
     Fused5Spec spec;
-    float *M;  // device copy of the 64x64 real block matrix
+    float *M;  // device copy of the block's three real 32x32 matrices (Ur, Ui, Ur + Ui)
 };
 
+constexpr int kFused5MatrixFloats = 3 * 32 * 32;
+
 // The kernel's form of a 5-qubit block (targs[0] = least significant bit of
-// u's index, u row-major 32x32): the spec, and the 64x64 real matrix
-// [[Ur, -Ui], [Ui, Ur]] written to hm.
+// u's index, u row-major 32x32): the spec, and the three real 32x32 matrices
+// Ur, Ui and Ur + Ui (row-major, one after the other) written to hm.
 void fused5Build(const int *targs, const Cplx *u, Fused5Spec &spec, float *hm) {
     // index the block by ascending target: bit j of a combination is the j-th
     // lowest target, so a wave's 4 combinations (MFMA K lanes) run over the
@@ -154,10 +170,9 @@ void fused5Build(const int *targs, const Cplx *u, Fused5Spec &spec, float *hm) {
     for (int r = 0; r < 32; ++r)
         for (int c = 0; c < 32; ++c) {
             const Cplx z = u[toOld[r] * 32 + toOld[c]];
-            hm[r * 64 + c] = (float)z.re;
-            hm[r * 64 + 32 + c] = (float)-z.im;
-            hm[(32 + r) * 64 + c] = (float)z.im;
-            hm[(32 + r) * 64 + 32 + c] = (float)z.re;
+            hm[r * 32 + c] = (float)z.re;
+            hm[1024 + r * 32 + c] = (float)z.im;
+            hm[2048 + r * 32 + c] = (float)(z.re + z.im);
         }
 }
 
@@ -176,7 +191,7 @@ void launchFused5(QuregDev *q, const Fused5Spec &spec, const float *M) {
 // many times (circuits).
 void *fused5Prepare(const int *targs, const Cplx *u) {
     auto *p = new Fused5Prepared;
-    std::vector<float> hm(64 * 64);
+    std::vector<float> hm(kFused5MatrixFloats);
     fused5Build(targs, u, p->spec, hm.data());
     p->M = static_cast<float *>(devAlloc(hm.size() * sizeof(float)));
     hostToDevice(p->M, hm.data(), hm.size() * sizeof(float));
@@ -186,7 +201,7 @@ void *fused5Prepare(const int *targs, const Cplx *u) {
 // One application of a block (multiQubitUnitary on an FP32 register): the
 // matrix goes through the staging ring, with no allocation or synchronisation.
 void fused5ApplyOnce(QuregDev *q, const int *targs, const Cplx *u) {
-    constexpr size_t bytes = 64 * 64 * sizeof(float);
+    constexpr size_t bytes = kFused5MatrixFloats * sizeof(float);
     static_assert(bytes <= kStageSlotBytes, "a fused block fits a staging slot");
     Fused5Spec spec;
     hipStream_t st = hstream();

@@ -77,6 +77,16 @@ void diagToDevice(qreal *dev, const qreal *host, size_t n) {
     qmx::hostToDevice(devDiag(dev), d.get(), n * sizeof(double));
 }
 
+void diagToHost(qreal *host, qreal *dev, size_t n) {
+    if constexpr (std::is_same<qreal, double>::value) {
+        qmx::deviceToHost(host, devDiag(dev), n * sizeof(double));
+    } else {
+        std::vector<double> d(n);
+        qmx::deviceToHost(d.data(), devDiag(dev), n * sizeof(double));
+        for (size_t i = 0; i < n; ++i) host[i] = (qreal)d[i];
+    }
+}
+
 uint64_t bitMask(const int *qubits, int n) {
     uint64_t m = 0;
     for (int i = 0; i < n; ++i) m |= 1ull << qubits[i];
@@ -1917,9 +1927,8 @@ static void diagFromHamil(DiagonalOp op, const PauliHamil &h) {
     std::vector<double> c(h.termCoeffs, h.termCoeffs + h.numSumTerms);
     qmx::diagonalFromZHamil(devDiag(op.deviceOperator.real), devDiag(op.deviceOperator.imag), op.numQubits, z.data(), c.data(),
                             h.numSumTerms);
-    const size_t bytes = (size_t)op.numElemsPerChunk * sizeof(qreal);
-    qmx::deviceToHost(op.real, op.deviceOperator.real, bytes);
-    qmx::deviceToHost(op.imag, op.deviceOperator.imag, bytes);
+    diagToHost(op.real, op.deviceOperator.real, (size_t)op.numElemsPerChunk);
+    diagToHost(op.imag, op.deviceOperator.imag, (size_t)op.numElemsPerChunk);
 }
 
 void initDiagonalOpFromPauliHamil(DiagonalOp op, PauliHamil hamil) {
@@ -1998,11 +2007,13 @@ static void multiVarImpl(const Qureg &qureg, int *qubits, int *numQubitsPerReg, 
                          qreal *coeffs, qreal *exponents, int *numTermsPerReg, long long *overrideInds,
                          qreal *overridePhases, int numOverrides, bool withOverrides, const char *caller) {
     qv::qubitSubregs(qureg, qubits, numQubitsPerReg, numRegs, caller);
+    qv::multiRegBitEncoding(numQubitsPerReg, numRegs, encoding, caller);
+    // the term counts are read only after the checks QuEST makes before them
     size_t totalTerms = 0;
-    for (int r = 0; r < numRegs; ++r) totalTerms += (size_t)std::max(0, numTermsPerReg[r]);
+    if (numTermsPerReg)
+        for (int r = 0; r < numRegs; ++r) totalTerms += (size_t)std::max(0, numTermsPerReg[r]);
     const Doubles c(coeffs, totalTerms), e(exponents, totalTerms);
     const Doubles ph(overridePhases, (size_t)std::max(0, numOverrides));
-    qv::multiRegBitEncoding(numQubitsPerReg, numRegs, encoding, caller);
     qv::multiVarPhaseFuncTerms(numRegs, encoding, e.get(), numTermsPerReg, caller);
     if (withOverrides)
         qv::multiVarPhaseFuncOverrides(numQubitsPerReg, numRegs, encoding, overrideInds, numOverrides, caller);

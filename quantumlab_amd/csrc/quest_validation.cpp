@@ -238,11 +238,14 @@ static bool isComplexPairUnitary(Complex a, Complex b) {
     return std::fabs(-1 + a.real * a.real + a.imag * a.imag + b.real * b.real + b.imag * b.imag) < REAL_EPS;
 }
 
+// The sums run in qreal, as QuEST's own (QuEST_validation.c:270-343), so that a
+// QuEST_PREC=1 build accepts and rejects exactly the matrices and maps QuEST's
+// float build does: near REAL_EPS = 1e-5 the float rounding of the sums decides.
 template <typename GetRe, typename GetIm>
 static bool isUnitary(int dim, GetRe re, GetIm im) {
     for (int r = 0; r < dim; ++r)
         for (int c = 0; c < dim; ++c) {
-            double er = 0, ei = 0;
+            qreal er = 0, ei = 0;
             for (int i = 0; i < dim; ++i) {
                 er += re(r, i) * re(c, i) + im(r, i) * im(c, i);
                 ei += im(r, i) * re(c, i) - re(r, i) * im(c, i);
@@ -269,13 +272,13 @@ template <typename GetRe, typename GetIm>
 static bool isCPTP(int numOps, int dim, GetRe re, GetIm im) {
     for (int r = 0; r < dim; ++r)
         for (int c = 0; c < dim; ++c) {
-            double er = 0, ei = 0;
+            qreal er = 0, ei = 0;
             for (int n = 0; n < numOps; ++n)
                 for (int k = 0; k < dim; ++k) {
                     er += re(n, k, r) * re(n, k, c) + im(n, k, r) * im(n, k, c);
                     ei += re(n, k, r) * im(n, k, c) - im(n, k, r) * re(n, k, c);
                 }
-            double dist = std::fabs(ei) + std::fabs(er - ((r == c) ? 1 : 0));
+            const qreal dist = std::fabs(ei) + std::fabs(er - ((r == c) ? 1 : 0));
             if (dist > REAL_EPS) return false;
         }
     return true;
