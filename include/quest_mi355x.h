@@ -72,6 +72,16 @@ void qmxAmplitudeEncodeReal(Qureg qureg, const qreal *vec, int numQubits);
  * (the loop's m_iMeasureUntil mode). Returns the number of shots drawn.
  * At most 26 qubits. */
 int qmxSampleShots(Qureg qureg, const int *qubits, int numQubits, int shots, int untilCount, int *outcomes);
+/* calcProbOfAllOutcomes (QuEST.h, QuEST.c:1185-1192) without allocating any
+ * device memory. No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc). Its scratch is a fixed 8 MiB device array per GPU
+ * (2^20 doubles, held until the process ends), so it serves up to 2^19
+ * outcomes (fewer when many of the qubits are below qubit 6 of a large
+ * register); larger requests are rejected with QuEST's error convention. Calls
+ * that overlap on one GPU share that array and race. Same results as
+ * calcProbOfAllOutcomes (the same kernels). */
+void qmxCalcProbOfAllOutcomesNoAlloc(qreal *outcomeProbs, Qureg qureg, int *qubits, int numQubits);
 /* max |a_i - b_i| over amplitudes (any precisions, same size) */
 qreal qmxMaxAbsDiff(Qureg a, Qureg b);
 /* FP64 norm of the register; if rescale != 0 the register is renormalised */

@@ -284,6 +284,7 @@ def _make_abi(qreal) -> types.SimpleNamespace:
         "qmxTimerSlotMs": (C.c_double, [C.c_int]),
         "qmxMaxAbsDiff": (qreal, [Qureg, Qureg]),
         "qmxRenormalise": (qreal, [Qureg, C.c_int]),
+        "qmxCalcProbOfAllOutcomesNoAlloc": (None, [C.POINTER(qreal), Qureg, C.POINTER(C.c_int), C.c_int]),
         "qmxSaveStateCSV": (C.c_int, [Qureg, C.c_char_p]),
         "qmxLoadStateCSV": (C.c_longlong, [Qureg, C.c_char_p]),
         "reportState": (None, [Qureg]),
