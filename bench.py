@@ -52,6 +52,9 @@ sys.path.insert(0, ROOT)
 METRIC = "GB/s+gates/s at 32q FP64; swap-tests/s batched; Trotter solve time at F≥1-1e-4"
 HBM_PEAK_GBS = 8000.0
 FP32_MATRIX_PEAK_TFLOPS = 157.3
+# FP32 FMA on the VALU without packed instructions: half of the published
+# 157.3 TFLOP/s vector peak, which counts packed FP32 (not used here)
+FP32_VALU_PEAK_TFLOPS = 78.6
 PERIOD = 5
 
 
@@ -344,12 +347,20 @@ def cpu_density_baseline(n_cpu, layers, seed):
                       f"OpenMP {threads} threads; {dt:.2f} s"}
 
 
-def run_trotter(qe, n, count, seed, eps=1e-4):
+def clopper_pearson_lower(successes, trials, alpha=0.05):
+    """One-sided (1 - alpha) lower confidence bound on a success probability
+    (Clopper-Pearson, exact binomial)."""
+    from scipy.stats import beta
+    return 0.0 if successes == 0 else float(beta.ppf(alpha, successes, trials - successes + 1))
+
+
+def run_trotter(qe, n, count, seed, eps=1e-4, sd_count=16):
     """Hamiltonian simulation at n qubits (FP32 registers): per instance, the
     learned planner's (order, reps) and one applyTrotterCircuit, timed from the
-    Hamiltonian to the evolved state (features, network, evolution); its
-    normalised infidelity against the 4x-finer FP64 run at the same order;
-    and plain step doubling at fixed order 2 and 4, timed the same way."""
+    Hamiltonian to the evolved state (features, network, evolution), and its
+    normalised infidelity against the 4x-finer FP64 run at the same order; on
+    the first sd_count instances also plain step doubling at fixed order 2 and
+    4, timed the same way (the speed-up compares the same instances)."""
     import ctypes as C
 
     import numpy as np
@@ -376,7 +387,7 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
         infid = 1 - (ip.real ** 2 + ip.imag ** 2) / (qe.calcTotalProb(ref) * qe.calcTotalProb(q))
         qe.destroy(ref)
         sd = {}
-        for order in (2, 4):
+        for order in ((2, 4) if idx < sd_count else ()):
             H.prepare(qe, q, inst)
             qe.qmxSync()
             reps = C.c_int()
@@ -387,21 +398,31 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
         qe.destroy(q)
         sweeps = qe.lib.qmxTrotterSweepsPerRep(hb[0], o.value, 0, QMX_PREC_FP32) * r.value
         rows.append((inst, o.value, r.value, t_learned, infid, sd, sweeps))
-        log(f"trotter {idx}: {inst.model} {inst.terms} terms t={inst.time:.2f}: learned order {o.value} x{r.value} "
-            f"{t_learned * 1e3:.0f} ms (1-F {infid:.1e}); step doubling o2 x{sd[2][1]} {sd[2][0] * 1e3:.0f} ms, "
-            f"o4 x{sd[4][1]} {sd[4][0] * 1e3:.0f} ms")
+        if sd:
+            log(f"trotter {idx}: {inst.model} {inst.terms} terms t={inst.time:.2f}: learned order {o.value} "
+                f"x{r.value} {t_learned * 1e3:.0f} ms (1-F {infid:.1e}); step doubling o2 x{sd[2][1]} "
+                f"{sd[2][0] * 1e3:.0f} ms, o4 x{sd[4][1]} {sd[4][0] * 1e3:.0f} ms")
+        elif idx % 16 == 15:
+            log(f"trotter {idx}: learned order {o.value} x{r.value} {t_learned * 1e3:.0f} ms (1-F {infid:.1e})")
     net.close()
-    tl = np.array([x[3] for x in rows])
-    t2 = np.array([x[5][2][0] for x in rows])
-    t4 = np.array([x[5][4][0] for x in rows])
+    tl_all = np.array([x[3] for x in rows])
     inf = np.array([x[4] for x in rows])
+    succ = int(np.sum(inf <= eps))
+    timed = [x for x in rows if x[5]]
+    tl = np.array([x[3] for x in timed])
+    t2 = np.array([x[5][2][0] for x in timed])
+    t4 = np.array([x[5][4][0] for x in timed])
     exps = sum(H.stages(x[1]) * x[2] * x[0].terms for x in rows)
     sweep_bytes = sum(x[6] for x in rows) * 2.0 * (1 << n) * 8
-    gbs = sweep_bytes / tl.sum() / 1e9
+    gbs = sweep_bytes / tl_all.sum() / 1e9
     return {"metric": "Trotter solve time at F>=1-1e-4", "unit": "ms", "higher_is_better": False,
             "workload": f"{count} seeded Hamiltonians (TFIM/Heisenberg chains + 32 random 2-4-local strings) on "
-                        f"{n} qubits, random product states, t ~ U(0.5, 4); FP32 registers",
+                        f"{n} qubits, random product states, t ~ U(0.5, 4); FP32 registers; none of them in the "
+                        f"planner's training or calibration sets (other seeds)",
             "qubits": n, "instances": count, "dtype": "f32",
+            "successes": succ, "success_lower_95": round(clopper_pearson_lower(succ, count), 4),
+            "step_doubling_instances": len(timed),
+            "learned_ms_mean_all": round(float(tl_all.mean() * 1e3), 2),
             "learned_ms_mean": round(float(tl.mean() * 1e3), 2),
             "step_doubling_o2_ms_mean": round(float(t2.mean() * 1e3), 2),
             "step_doubling_o4_ms_mean": round(float(t4.mean() * 1e3), 2),
@@ -409,8 +430,8 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
             "speedup_vs_step_doubling_o4": round(float(t4.sum() / tl.sum()), 3),
             "speedup_vs_best_fixed_order": round(float(min(t2.sum(), t4.sum()) / tl.sum()), 3),
             "success_rate": float(np.mean(inf <= eps)), "max_infidelity_vs_4x_finer_fp64": float(inf.max()),
-            "step_doubling_met_eps": bool(all(x[5][2][2] and x[5][4][2] for x in rows)),
-            "pauli_exponentials_per_s_learned": round(exps / tl.sum(), 1),
+            "step_doubling_met_eps": bool(all(x[5][2][2] and x[5][4][2] for x in timed)),
+            "pauli_exponentials_per_s_learned": round(exps / tl_all.sum(), 1),
             "roofline": {"bound": "hbm", "kernel": "kPauliCoset<float> (fused Pauli-rotation sweeps)",
                          "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
@@ -420,7 +441,7 @@ def run_trotter(qe, n, count, seed, eps=1e-4):
                                  "fewer, fuller blocks, so a sweep carries more rotations (and LDS round trips, one "
                                  "per coset group) per HBM read + write"},
             "choices": [[x[1], x[2]] for x in rows],
-            "step_doubling_reps": [[x[5][2][1], x[5][4][1]] for x in rows]}, rows
+            "step_doubling_reps": [[x[5][2][1], x[5][4][1]] for x in timed]}, rows
 
 
 def cpu_trotter_baseline(rows, n):
@@ -511,6 +532,7 @@ def run_swap_tests(qe, points, centroids, iterations, warmup, seed):
     blocks = int(qe.lib.qmxSwapPlanMfmaBlocks(plan.plan))
     tile_ms = float(qe.lib.qmxSwapPlanTimeTiles(plan.plan, 5))
     tflops = blocks * 4096 / (tile_ms / 1e3) / 1e12
+    valu_tflops = blocks * 2048 / (tile_ms / 1e3) / 1e12
     evals = K.swap_test_amplitude_evaluations(data)
     res = {"swap_tests_per_s": round(tests * iterations / wall, 1), "unit": "swap-tests/s",
            "workload": f"{points} points x {centroids} centroids, q in 4..10 (registers 2q+1 = 9..21 qubits), "
@@ -522,13 +544,20 @@ def run_swap_tests(qe, points, centroids, iterations, warmup, seed):
            "max_distance_error_vs_exact_fp64": dist_err, "max_distance_error_fp64_plan": dist_err64,
            "timed_run_assignments_identical_to_fp64": timed_same, "timed_run_max_distance_diff_vs_fp64": timed_err,
            "autotune_timings": timings, "rows_per_tile": rows, "tiles": plan.num_tiles(),
-           "roofline": {"bound": "mfma", "kernel": "kSwapTilesMfma (32x32x2 FP32 MFMA blocks of the amplitude matrix)",
-                        "achieved": round(tflops, 2), "peak": FP32_MATRIX_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": round(tflops / FP32_MATRIX_PEAK_TFLOPS, 4), "traffic": None,
+           "method": "closed-form evaluation of the circuit's ancilla-0 amplitudes (a_x b_y + a_y b_x)/2 over all "
+                     "4^q index pairs, from the encoded registers; the CSWAP ladder and the Hadamards are not applied "
+                     "gate by gate (the gate-level circuit is the oracle's and the cpu_reference_method's)",
+           "roofline": {"bound": "valu", "kernel": "kSwapTilesMfma (32x32x2 FP32 MFMA blocks of the amplitude matrix, "
+                                                   "squares summed on the VALU)",
+                        "achieved": round(valu_tflops, 2), "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(valu_tflops / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
                         "avg_launch_ms": round(tile_ms, 4), "mfma_blocks_per_launch": blocks,
-                        "flops_per_block": 4096,
+                        "valu_flops_per_block": 2048, "mfma_tflops": round(tflops, 2),
+                        "mfma_frac": round(tflops / FP32_MATRIX_PEAK_TFLOPS, 4),
                         "note": "registers are L2-resident (19 MB of FP32 point registers), so HBM does not bound it; "
-                                "each block also costs 16 FP32 FMAs per lane for the squares"}}
+                                "the matrix cores form each 32x32 block of amplitudes (4096 flops) and the VALU "
+                                "squares and sums its 1024 entries (one FP32 FMA each, 2048 flops), so the VALU, "
+                                "at its non-packed FP32 rate (packed FP32 is not used), is the bound"}}
     plan.close()
     p64.close()
     return res, data
@@ -697,16 +726,20 @@ def main():
     ap.add_argument("--no-density", action="store_true")
     ap.add_argument("--no-trotter", action="store_true")
     ap.add_argument("--trotter-qubits", type=int, default=24)
-    ap.add_argument("--trotter-count", type=int, default=8)
+    ap.add_argument("--trotter-count", type=int, default=128)
+    ap.add_argument("--trotter-step-doubling", type=int, default=16,
+                    help="instances (the first ones) that also run the step-doubling baselines")
     ap.add_argument("--density-qubits", type=int, default=14)
     ap.add_argument("--density-layers", type=int, default=60)
     ap.add_argument("--cpu-density-qubits", type=int, default=13)
     ap.add_argument("--swap-points", type=int, default=16384)
     ap.add_argument("--swap-centroids", type=int, default=16)
     ap.add_argument("--swap-iterations", type=int, default=10)
-    ap.add_argument("--only", choices=["statevec", "fused", "fused-large", "density", "trotter", "swap", "stream"],
+    ap.add_argument("--only", choices=["statevec", "fused", "fused-large", "fused-kernel", "density", "trotter", "swap",
+                                       "stream"],
                     help="run one leg (for profiling it alone); 'fused' includes the FP64 per-gate run it is "
-                         "compared with, the others skip the headline leg; no CPU baselines")
+                         "compared with, 'fused-kernel' is the fused FP32 periods alone, the others skip the "
+                         "headline leg; no CPU baselines")
     args = ap.parse_args()
     if args.only:
         args.no_cpu = True
@@ -717,6 +750,7 @@ def main():
         if args.only == "fused-large":
             args.fused_large_only = True
         elif args.only not in ("statevec", "fused"):
+            args.fused_steps = args.steps
             args.steps = 0
         if args.only == "statevec":
             args.fp32_qubits = 0
@@ -781,13 +815,21 @@ def main():
 
     if args.only == "stream":
         result["stream"] = run_stream(qe, n)
+    if args.only == "fused-kernel":
+        plans = [period_gates(n, s, args.seed) for s in range(args.warmup + args.fused_steps)]
+        q32, wall32, blocks, _ = run_fp32_fused(qe, n, plans, args.warmup, args.fused_steps, args.fuse_qubits)
+        qe.destroy(q32)
+        gbs = blocks * 2.0 * N * 8 / wall32 / 1e9
+        result["fused_kernel"] = {"qubits": n, "periods": args.fused_steps, "blocks": blocks,
+                                  "ms_per_block": round(wall32 / blocks * 1e3, 3), "GBps": round(gbs, 1)}
 
     if not args.no_density:
         result["density"] = run_density(qe, args.density_qubits, args.density_layers, args.seed)
 
     trotter_rows = None
     if not args.no_trotter:
-        result["trotter"], trotter_rows = run_trotter(qe, args.trotter_qubits, args.trotter_count, args.seed)
+        result["trotter"], trotter_rows = run_trotter(qe, args.trotter_qubits, args.trotter_count, args.seed,
+                                                    sd_count=args.trotter_step_doubling)
         tr = result["trotter"]
         log(f"trotter: learned {tr['learned_ms_mean']} ms vs step doubling o2 {tr['step_doubling_o2_ms_mean']} ms, "
             f"o4 {tr['step_doubling_o4_ms_mean']} ms; success {tr['success_rate']:.2f}")

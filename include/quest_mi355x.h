@@ -62,6 +62,33 @@ void qmxDensityFusionStats(long long *sweeps, long long *operations);
  * in FP64), in one launch instead of 2^numQubits gates. State vectors, at most
  * 12 encoded qubits; the register's other qubits are left |0>. */
 void qmxAmplitudeEncodeReal(Qureg qureg, const qreal *vec, int numQubits);
+/* QuantumLab's complex-vector encoder for v = re + i im (AmplitudeEncodeOneVector
+ * with bHasPhase false, AmplitudeEncodeCuda.cu:691-724: the real encoder's
+ * magnitude tree over |v_i| plus a phase tree over arg v_i, applied as Ry and
+ * Rz(-Z) per level, ExchangeToYZGate :452-496 with FRyz) applied to |0...0>:
+ * the register ends in v/|v| times exp(-i m), m the mean of the arg v_i (the
+ * global phase the reference leaves to an EBO_Phase gate, which its QuEST
+ * simulators do not apply). Angles and product in FP64, one launch; state
+ * vectors, at most 12 encoded qubits (qubits 0..numQubits-1, bit l <-> qubit
+ * l); the other qubits are left |0>. */
+void qmxAmplitudeEncodeComplex(Qureg qureg, const qreal *re, const qreal *im, int numQubits);
+/* A uniformly controlled (multiplexed) one-qubit gate in one sweep: wherever the
+ * control qubits read k (bit b of k = qubit controls[b]), ops[k] acts on the
+ * target. 2^numControls unitary matrices, 0 to 16 controls. State vectors and
+ * density matrices (ops[k] on the rows, its conjugate on the columns). */
+void qmxUniformlyControlledUnitary(Qureg qureg, const int *controls, int numControls, int target,
+                                   const ComplexMatrix2 *ops);
+enum qmxUniformGate { QMX_UNIFORM_RY = 2, QMX_UNIFORM_RZ = 3, QMX_UNIFORM_PHASE = 4 };
+/* QuantumLab's uniformly controlled rotations FRy, FRz and FRp (FRGate.cu:
+ * 181-330, 460-515), which it builds from 2^numControls rotations and CNOTs
+ * (or X flips and multi-controlled phases), as one sweep: where the controls
+ * read k, Ry(angles[k]) (rotateY's convention), Rz(angles[k]) (rotateZ's) or
+ * the phase shift diag(1, e^{i angles[k]}) acts on the target. FRy(angles, n)
+ * or FRz on register qubits b[0..n-1] is this call with target b[n-1] and
+ * controls[j] = b[n-2-j]; FRp with controls[j] = b[j]. FRyz(Y, Z) is RY with
+ * Y, then RZ with -Z. */
+void qmxUniformlyControlledRotation(Qureg qureg, const int *controls, int numControls, int target,
+                                    const qreal *angles, enum qmxUniformGate kind);
 /* Repeated measurement of qubits[0..numQubits) in that order, as a loop that
  * restores the register and calls measure() on each qubit per shot would do
  * (QuantumLab's QLSimulatorMeasure.cpp:113-140), without touching the state:

@@ -137,6 +137,166 @@ __global__ void __launch_bounds__(kBlock, 5) kFused5(Amp<float> *__restrict__ a,
     }
 }
 
+// ---- tile form (registers of 11 qubits and more) -----------------------------
+// A workgroup owns tiles of 64 groups x 32 combinations = 2048 amplitudes
+// (16 KiB). A tile's index bits are P = the 5 targets and the 6 lowest other
+// qubits, so P always holds qubits 0-5 and the tile is a union of contiguous
+// runs of at least 64 amplitudes (512 B): it moves between HBM and LDS as
+// coalesced nontemporal 16 B accesses whatever the targets, like the in-place
+// streams. Each wave multiplies 16 groups out of LDS (the MFMA forms above) and
+// writes the results back in place; the workgroup then stores the tile. The
+// workgroups are persistent: the matrix fragments are read once into registers,
+// and the next tile's loads are in flight while the current one multiplies
+// and is stored. LDS holds two tiles (double buffering, one barrier fewer).
+//
+// LDS slot of tile element e: a linear XOR swizzle (slotBit[i] = slot of
+// element 2^i) chosen on the host so that the 16 lanes of an MFMA operand read
+// (one combination, 16 consecutive groups) hit 16 different 8 B bank pairs
+// whichever bit positions the groups take.
+constexpr int kTileLog = 11;
+constexpr int kTileGroups = 64;
+constexpr int kTileAmps = 32 * kTileGroups;
+
+struct Fused5TileSpec {
+    int pos[kTileLog];          // P, ascending
+    int tIdx[5];                // index in P of the j-th lowest target (combination bit j)
+    int gIdx[6];                // index in P of group bit b
+    unsigned slotBit[kTileLog];  // swizzle: slot of element 2^i
+    long long numTiles, tilesPerGroup;
+};
+
+__device__ __forceinline__ unsigned tileSlot(unsigned e, const Fused5TileSpec &sp) {
+    unsigned s = 0;
+#pragma unroll
+    for (int i = 0; i < kTileLog; ++i)
+        if ((e >> i) & 1) s ^= sp.slotBit[i];
+    return s;
+}
+
+__device__ __forceinline__ unsigned tileCombo(unsigned c, const Fused5TileSpec &sp) {
+    unsigned e = 0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) e |= ((c >> j) & 1u) << sp.tIdx[j];
+    return e;
+}
+
+// state index of tile t's element 0: t with zeros inserted at the P positions
+__device__ __forceinline__ uint64_t tileBase(uint64_t t, const Fused5TileSpec &sp) {
+#pragma unroll
+    for (int i = 0; i < kTileLog; ++i) t = insertZero(t, sp.pos[i]);
+    return t;
+}
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict__ a, Fused5TileSpec sp,
+                                                      const float *__restrict__ M) {
+    __shared__ __attribute__((aligned(16))) Amp<float> buf[2][kTileAmps];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const long long first = xcdBlock() * sp.tilesPerGroup;
+    const long long last = min(first + sp.tilesPerGroup, sp.numTiles);
+    if (first >= last) return;  // the same for every thread of the workgroup
+
+    // matrix fragments, once: product p, row tile h, step s -> M_p[16h + col][4s + kq]
+    float fr[2][3][8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int st = 0; st < 8; ++st) fr[h][p][st] = M[p * 1024 + (16 * h + col) * 32 + 4 * st + kq];
+
+    // LDS slots of this lane's operands: group g = 16 wave + col; input
+    // combination 4s + kq, output combination 16h + 4kq + v (linear swizzle:
+    // slot(x | y) = slot(x) ^ slot(y) for disjoint bits)
+    const unsigned g = 16 * wave + col;
+    unsigned eG = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) eG |= ((g >> b) & 1u) << sp.gIdx[b];
+    const unsigned inLane = tileSlot(eG | tileCombo(kq, sp), sp);
+    const unsigned outLane = tileSlot(eG | tileCombo(4 * kq, sp), sp);
+    unsigned sIn[8], sOut[8];
+#pragma unroll
+    for (int st = 0; st < 8; ++st) sIn[st] = tileSlot(tileCombo(4 * st, sp), sp);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sOut[4 * h + v] = tileSlot(tileCombo(16 * h + v, sp), sp);
+
+    // coalesced transfers: thread tid moves elements e = 2 tid + 512 j and e + 1
+    // (j < 4); bits 0-5 of e are qubits 0-5, bits 6-10 go to P[6..10]. The
+    // parts from tid (bits 0-8) and from j (bits 9-10) are disjoint, so address
+    // and slot split into a per-lane and a uniform term.
+    const unsigned eLane = 2 * tid;
+    uint64_t laneAddr = eLane & 63u;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) laneAddr |= (uint64_t)((eLane >> (6 + i)) & 1u) << sp.pos[6 + i];
+    const unsigned laneSlot = tileSlot(eLane, sp);
+    uint64_t jAddr[4];
+    unsigned jSlot[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        jAddr[j] = ((uint64_t)(j & 1) << sp.pos[9]) | ((uint64_t)(j >> 1) << sp.pos[10]);
+        jSlot[j] = tileSlot(512u * j, sp);
+    }
+    const unsigned slot1 = sp.slotBit[0];  // slot(e + 1) = slot(e) ^ slot(1) for even e
+
+    f4v R[4];
+    uint64_t base = tileBase((uint64_t)first, sp);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        R[j] = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(a + ((base | laneAddr) | jAddr[j])));
+    int b = 0;
+    for (long long t = first; t < last; ++t, b ^= 1) {
+        Amp<float> *T = buf[b];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            T[laneSlot ^ jSlot[j]] = Amp<float>{R[j].x, R[j].y};
+            T[laneSlot ^ jSlot[j] ^ slot1] = Amp<float>{R[j].z, R[j].w};
+        }
+        __syncthreads();
+        const uint64_t cur = base;
+        if (t + 1 < last) {
+            base = tileBase((uint64_t)(t + 1), sp);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                R[j] = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(a + ((base | laneAddr) | jAddr[j])));
+        }
+        // both row tiles accumulate together, so an operand is live for one step
+        f4 acc[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int p = 0; p < 3; ++p) acc[h][p] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            const Amp<float> x = T[inLane ^ sIn[st]];
+            const float xs = x.x + x.y;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                acc[h][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr[h][0][st], x.x, acc[h][0], 0, 0, 0);
+                acc[h][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr[h][1][st], x.y, acc[h][1], 0, 0, 0);
+                acc[h][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr[h][2][st], xs, acc[h][2], 0, 0, 0);
+            }
+        }
+        // the wave's reads of these 16 groups are done: results go back in place
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                T[outLane ^ sOut[4 * h + v]] = Amp<float>{acc[h][0][v] - acc[h][1][v],
+                                                          acc[h][2][v] - acc[h][0][v] - acc[h][1][v]};
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const Amp<float> lo = T[laneSlot ^ jSlot[j]], hi = T[laneSlot ^ jSlot[j] ^ slot1];
+            __builtin_nontemporal_store(f4v{lo.x, lo.y, hi.x, hi.y},
+                                        reinterpret_cast<f4v *>(a + ((cur | laneAddr) | jAddr[j])));
+        }
+    }
+}
+
 struct Fused5Prepared {
     Fused5Spec spec;
     float *M;  // device copy of the block's three real 32x32 matrices (Ur, Ui, Ur + Ui)
@@ -176,7 +336,62 @@ void fused5Build(const int *targs, const Cplx *u, Fused5Spec &spec, float *hm) {
         }
 }
 
+// The tile form's index positions and LDS swizzle for a block whose ascending
+// targets are t[0..4].
+Fused5TileSpec tileSpecOf(const BitList &targets, long long numAmps) {
+    Fused5TileSpec sp{};
+    uint64_t targ = 0;
+    for (int j = 0; j < 5; ++j) targ |= 1ull << targets.pos[j];
+    int groupPos[6], ng = 0;
+    for (int p = 0; ng < 6; ++p)
+        if (!((targ >> p) & 1)) groupPos[ng++] = p;
+    uint64_t pm = targ;
+    for (int b = 0; b < 6; ++b) pm |= 1ull << groupPos[b];
+    int k = 0;
+    for (int p = 0; p < 64 && k < kTileLog; ++p)
+        if ((pm >> p) & 1) sp.pos[k++] = p;
+    auto indexOf = [&](int position) {
+        for (int i = 0; i < kTileLog; ++i)
+            if (sp.pos[i] == position) return i;
+        return -1;
+    };
+    for (int j = 0; j < 5; ++j) sp.tIdx[j] = indexOf(targets.pos[j]);
+    for (int b = 0; b < 6; ++b) sp.gIdx[b] = indexOf(groupPos[b]);
+    // swizzle: group bits 0-3 (the lanes of one operand read) onto distinct slot
+    // bits 0-3; those already below 4 stay, the others take the free low bits
+    for (int i = 0; i < kTileLog; ++i) sp.slotBit[i] = 1u << i;
+    bool used[4] = {false, false, false, false};
+    for (int b = 0; b < 4; ++b)
+        if (sp.gIdx[b] < 4) used[sp.gIdx[b]] = true;
+    for (int b = 0; b < 4; ++b) {
+        if (sp.gIdx[b] < 4) continue;
+        int f = 0;
+        while (used[f]) ++f;
+        used[f] = true;
+        sp.slotBit[sp.gIdx[b]] |= 1u << f;
+    }
+    sp.numTiles = numAmps / kTileAmps;
+    return sp;
+}
+
 void launchFused5(QuregDev *q, const Fused5Spec &spec, const float *M) {
+    if (q->numQubits >= kTileLog) {
+        Fused5TileSpec sp = tileSpecOf(spec.targetsSorted, q->numAmps);
+        // persistent workgroups: 3 per CU (the kernel's occupancy), a multiple of 8
+        // (one share per XCD)
+        static long long grid = 0;
+        if (!grid) {
+            int cus = 256;
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
+            grid = std::max(8, (cus * 3) / 8 * 8);
+        }
+        const long long g = std::min<long long>(grid, (sp.numTiles + 7) / 8 * 8);
+        sp.tilesPerGroup = (sp.numTiles + g - 1) / g;
+        hipLaunchKernelGGL(kFused5Tile, dim3((unsigned)g), dim3(kBlock), 0, hstream(), (Amp<float> *)q->amps, sp, M);
+        checkLaunch("fused5Apply");
+        return;
+    }
     const long long numChunks = (q->numAmps >> 5) / 16;
     const long long wavesNeeded = (numChunks + kChunksPerWave - 1) / kChunksPerWave;
     const long long grid = (wavesNeeded + (kBlock / 64) - 1) / (kBlock / 64);

@@ -98,6 +98,33 @@ __global__ void __launch_bounds__(kBlock) kMatrix2Span(Amp<T> *__restrict__ a, i
     }
 }
 
+// ---- uniformly controlled 2x2 ------------------------------------------------
+// Matrix k of a table acts on the target wherever the controls read k (bit b
+// of k = qubit ctrl[b]). QuantumLab builds the rotation forms as 2^c rotations
+// and CNOTs (FRy / FRz / FRyz, FRGate.cu:181-460) or X flips and
+// multi-controlled phases (FRp, FRGate.cu:460-515), one sweep each; here it is
+// one sweep in all. The table (2^c x 4 complex) is read through the caches:
+// neighbouring pairs share k unless a control is among the lowest bits.
+constexpr int kMaxUniformControls = 16;
+struct CtrlBits {
+    int n;
+    int pos[kMaxUniformControls];
+};
+
+template <typename T, bool NT>
+__global__ void __launch_bounds__(kBlock) kUniform2(Amp<T> *__restrict__ a, long long work, int target,
+                                                    CtrlBits ctrl, const M2<T> *__restrict__ mats) {
+    const long long w = flatBlock() * kBlock + threadIdx.x;
+    if (w >= work) return;
+    const uint64_t i0 = insertZero((uint64_t)w, target), i1 = i0 | (1ull << target);
+    int k = 0;
+    for (int b = 0; b < ctrl.n; ++b) k |= (int)((i0 >> ctrl.pos[b]) & 1) << b;
+    const M2<T> u = mats[k];
+    const Amp<T> up = ldA<T, NT>(a + i0), lo = ldA<T, NT>(a + i1);
+    stA<T, NT>(a + i0, cmul2(u.m00, up, u.m01, lo));
+    stA<T, NT>(a + i1, cmul2(u.m10, up, u.m11, lo));
+}
+
 // ---- permutations ----------------------------------------------------------
 // multiControlledMultiQubitNot (QuEST_cpu.c:2761-2810), controlledNot
 // (QuEST_cpu.c:2662-2710): swap i <-> i ^ targMask where the controls are 1
@@ -553,6 +580,46 @@ void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const
                            (Amp<T> *)q->amps, numTiles, ts, (const C2<T> *)dMat, matInLds);
     });
     checkLaunch("applyMatrixN");
+}
+
+// Uniformly controlled 2x2: u holds 4 entries (row-major) per control value.
+void applyUniform2(QuregDev *q, const int *controls, int numControls, int target, const Cplx *u) {
+    if (numControls > kMaxUniformControls) {
+        std::fprintf(stderr, "libquest_mi355x: applyUniform2 supports at most %d controls (got %d)\n",
+                     kMaxUniformControls, numControls);
+        std::abort();
+    }
+    CtrlBits cb{};
+    cb.n = numControls;
+    for (int b = 0; b < numControls; ++b) cb.pos[b] = controls[b];
+    const long long work = q->numAmps >> 1;
+    const bool nt = useNT(target, q->prec);
+    const size_t count = (size_t)1 << numControls;
+    QMX_DISPATCH(q->prec, T, {
+        const size_t bytes = count * sizeof(M2<T>);
+        hipStream_t st = hstream();
+        const void *dMat;
+        if (bytes <= kStageSlotBytes) {
+            const Staged sg = stageBegin(bytes);
+            M2<T> *h = static_cast<M2<T> *>(sg.host);
+            for (size_t k = 0; k < count; ++k)
+                h[k] = M2<T>{c2<T>(u[4 * k]), c2<T>(u[4 * k + 1]), c2<T>(u[4 * k + 2]), c2<T>(u[4 * k + 3])};
+            stageCommit(sg, bytes, st);
+            dMat = sg.dev;
+        } else {
+            std::vector<M2<T>> h(count);
+            for (size_t k = 0; k < count; ++k)
+                h[k] = M2<T>{c2<T>(u[4 * k]), c2<T>(u[4 * k + 1]), c2<T>(u[4 * k + 2]), c2<T>(u[4 * k + 3])};
+            dMat = stageLarge(h.data(), bytes, st);
+        }
+        if (nt)
+            hipLaunchKernelGGL((kUniform2<T, true>), gridFor(work, 1), dim3(kBlock), 0, st, (Amp<T> *)q->amps, work,
+                               target, cb, (const M2<T> *)dMat);
+        else
+            hipLaunchKernelGGL((kUniform2<T, false>), gridFor(work, 1), dim3(kBlock), 0, st, (Amp<T> *)q->amps,
+                               work, target, cb, (const M2<T> *)dMat);
+    });
+    checkLaunch("applyUniform2");
 }
 
 }  // namespace qmx

@@ -115,6 +115,8 @@ void applyMultiRotateZ(QuregDev *q, uint64_t ctrlMask, uint64_t targMask, double
 // dense 2^k x 2^k matrix (row-major, u[r*dim+c]) on targets (targs[0] = least
 // significant bit of the row index), applied where ctrl bits are 1
 void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const Cplx *u);
+// uniformly controlled 2x2: matrix k (4 entries, row-major) where the controls read k
+void applyUniform2(QuregDev *q, const int *controls, int numControls, int target, const Cplx *u);
 // dense 5-qubit block on an FP32 register via MFMA (qmx_fused.hip); prepared
 // once (device-resident matrix), applied to registers of >= 9 qubits
 void *fused5Prepare(const int *targs, const Cplx *u);
@@ -138,6 +140,7 @@ int pauliSequenceBlocks(int numQubits, int prec, const PauliExp *seq, int count)
 // amplitude-encoding circuit of `vec` (2^qbits reals, qbits <= 12) applied to
 // |0...0> of the register (qmx_swaptest.hip)
 void amplitudeEncode(QuregDev *q, const double *vec, int qbits);
+void amplitudeEncodeComplex(QuregDev *q, const double *re, const double *im, int qbits);
 
 // dense 2^B x 2^B matrix (row-major; index bit j <-> register bit bits[j]) on
 // B = 2..5 arbitrary bits, one group per thread in registers (qmx_dense.hip)

@@ -4,6 +4,7 @@
 // workload's hot path; they are one streaming pass each, like the gate kernels.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "qmx_device.h"
@@ -53,47 +54,68 @@ struct PhaseDev {
     const long long *overrideInds;
 };
 
-__device__ double namedPhase(const PhaseDev &p, const long long *x) {
+// The phase arithmetic runs in qreal with every operation rounded separately,
+// as QuEST's loops do (QuEST_cpu.c:4268-4323, 4347-4424, 4449-4584: qreal
+// phase, norm, prod and dist, integer sub-register values converted where C
+// converts them). In the float build (QuEST_PREC=1) phases reach 10^5 rad,
+// where a float step is 0.01 rad, so accumulating in double instead would move
+// a phase by more than QuEST's own tests allow against a float evaluation.
+using PhaseReal = qreal;
+
+__device__ PhaseReal namedPhase(const PhaseDev &p, const long long *x) {
+#pragma clang fp contract(off)
+    typedef PhaseReal R;
     const int f = p.funcCode;
-    const double *prm = p.params;
+    const double *prm = p.params;  // qreal values, held exactly in double
     if (f == NORM || f == INVERSE_NORM || f == SCALED_NORM || f == SCALED_INVERSE_NORM ||
         f == SCALED_INVERSE_SHIFTED_NORM) {
-        double norm = 0;
+        R norm = 0;
         for (int r = 0; r < p.numRegs; ++r) {
-            const double d = f == SCALED_INVERSE_SHIFTED_NORM ? (double)x[r] - prm[2 + r] : (double)x[r];
-            norm += d * d;
+            if (f == SCALED_INVERSE_SHIFTED_NORM) {
+                const R d = (R)x[r] - (R)prm[2 + r];
+                norm += d * d;
+            } else {
+                norm += (R)(x[r] * x[r]);
+            }
         }
-        norm = sqrt(norm);
+        norm = (R)sqrt((double)norm);
         if (f == NORM) return norm;
-        if (f == INVERSE_NORM) return norm == 0. ? prm[0] : 1 / norm;
-        if (f == SCALED_NORM) return prm[0] * norm;
-        return norm <= REAL_EPS ? prm[1] : prm[0] / norm;
+        if (f == INVERSE_NORM) return norm == 0. ? (R)prm[0] : (R)1 / norm;
+        if (f == SCALED_NORM) return (R)prm[0] * norm;
+        return norm <= REAL_EPS ? (R)prm[1] : (R)prm[0] / norm;
     }
     if (f == PRODUCT || f == INVERSE_PRODUCT || f == SCALED_PRODUCT || f == SCALED_INVERSE_PRODUCT) {
-        double prod = 1;
-        for (int r = 0; r < p.numRegs; ++r) prod *= (double)x[r];
+        R prod = 1;
+        for (int r = 0; r < p.numRegs; ++r) prod *= (R)x[r];
         if (f == PRODUCT) return prod;
-        if (f == INVERSE_PRODUCT) return prod == 0. ? prm[0] : 1 / prod;
-        if (f == SCALED_PRODUCT) return prm[0] * prod;
-        return prod == 0. ? prm[1] : prm[0] / prod;
+        if (f == INVERSE_PRODUCT) return prod == 0. ? (R)prm[0] : (R)1 / prod;
+        if (f == SCALED_PRODUCT) return (R)prm[0] * prod;
+        return prod == 0. ? (R)prm[1] : (R)prm[0] / prod;
     }
-    double dist = 0;  // distances between consecutive register pairs
+    R dist = 0;  // distances between consecutive register pairs
     for (int r = 0; r + 1 < p.numRegs; r += 2) {
-        const double d = f == SCALED_INVERSE_SHIFTED_DISTANCE ? (double)x[r] - (double)x[r + 1] - prm[2 + r / 2]
-                                                              : (double)x[r + 1] - (double)x[r];
-        dist += d * d;
+        if (f == SCALED_INVERSE_SHIFTED_DISTANCE) {
+            const R d = (R)(x[r] - x[r + 1]) - (R)prm[2 + r / 2];
+            dist += d * d;
+        } else {
+            dist += (R)((x[r + 1] - x[r]) * (x[r + 1] - x[r]));
+        }
     }
-    dist = sqrt(dist);
+    dist = (R)sqrt((double)dist);
     if (f == DISTANCE) return dist;
-    if (f == INVERSE_DISTANCE) return dist == 0. ? prm[0] : 1 / dist;
-    if (f == SCALED_DISTANCE) return prm[0] * dist;
-    return dist <= REAL_EPS ? prm[1] : prm[0] / dist;
+    if (f == INVERSE_DISTANCE) return dist == 0. ? (R)prm[0] : (R)1 / dist;
+    if (f == SCALED_DISTANCE) return (R)prm[0] * dist;
+    return dist <= REAL_EPS ? (R)prm[1] : (R)prm[0] / dist;
 }
 
 // one thread per amplitude: sub-register values, override lookup, phase,
 // amp *= exp(i phase) (QuEST_cpu.c:4251-4620)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) kPhaseFunc(Amp<T> *__restrict__ a, long long n, PhaseDev p) {
+#pragma clang fp contract(off)
+    typedef PhaseReal R;
+    // amplitudes update in float only when both the register and qreal are float
+    typedef typename std::conditional<sizeof(T) == 4 && sizeof(R) == 4, float, double>::type A;
     const long long i = flatBlockLinear() * kBlock + threadIdx.x;
     if (i >= n) return;
     long long x[kMaxRegs];
@@ -112,20 +134,22 @@ __global__ void __launch_bounds__(kBlock) kPhaseFunc(Amp<T> *__restrict__ a, lon
         for (int r = 0; r < p.numRegs && hit; ++r) hit = x[r] == p.overrideInds[(long long)o * p.numRegs + r];
         if (hit) break;
     }
-    double phase = 0;
+    R phase = 0;
     if (o < p.numOverrides) {
-        phase = p.overridePhases[o];
+        phase = (R)p.overridePhases[o];
     } else if (p.mode == 0) {
+        // phase += coeff * pow(x, exponent): the product in double, each sum rounded to qreal
         int t = 0;
         for (int r = 0; r < p.numRegs; ++r)
-            for (int k = 0; k < p.numTermsPerReg[r]; ++k, ++t) phase += p.coeffs[t] * pow((double)x[r], p.exponents[t]);
+            for (int k = 0; k < p.numTermsPerReg[r]; ++k, ++t)
+                phase = (R)((double)phase + p.coeffs[t] * pow((double)x[r], p.exponents[t]));
     } else {
         phase = namedPhase(p, x);
     }
     if (p.conj) phase = -phase;
-    const double c = cos(phase), s = sin(phase);
+    const A c = (A)(R)cos((double)phase), s = (A)(R)sin((double)phase);
     const Amp<T> v = a[i];
-    const double re = v.x, im = v.y;
+    const A re = v.x, im = v.y;
     a[i] = Amp<T>{(T)(re * c - im * s), (T)(re * s + im * c)};
 }
 

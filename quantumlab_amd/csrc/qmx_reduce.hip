@@ -56,14 +56,23 @@ __device__ __forceinline__ void writePartial(Acc (&acc)[NV], double *part) {
 template <typename T>
 __global__ void __launch_bounds__(kBlock) kSumSq(const Amp<T> *__restrict__ a, long long work,
                                                  BitList fixed, uint64_t setMask, double *part) {
+    // four loads in flight per thread before the compensated sums consume them
+    constexpr int U = 4;
     Acc acc[1];
-    for (long long w = (long long)blockIdx.x * kBlock + threadIdx.x; w < work;
-         w += (long long)gridDim.x * kBlock) {
-        uint64_t i = insertZeros((uint64_t)w, fixed) | setMask;
-        Amp<T> v = a[i];
-        double re = v.x, im = v.y;
-        acc[0].add(re * re);
-        acc[0].add(im * im);
+    const long long stride = (long long)gridDim.x * kBlock;
+    for (long long w0 = (long long)blockIdx.x * kBlock + threadIdx.x; w0 < work; w0 += U * stride) {
+        Amp<T> v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long w = w0 + u * stride;
+            v[u] = w < work ? ldA<T, true>(a + (insertZeros((uint64_t)w, fixed) | setMask)) : Amp<T>{0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double re = v[u].x, im = v[u].y;
+            acc[0].add(re * re);
+            acc[0].add(im * im);
+        }
     }
     writePartial<1>(acc, part);
 }

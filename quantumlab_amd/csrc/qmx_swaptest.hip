@@ -501,7 +501,82 @@ __global__ void __launch_bounds__(kBlock) kAEWrite(Amp<T> *__restrict__ a, const
     const int x = blockIdx.x * kBlock + threadIdx.x;
     if (x < D) a[x] = Amp<T>{(T)v[x], (T)0};
 }
+
+// AmplitudeEncodeOneVector without its global phase gate (AmplitudeEncodeCuda.cu:
+// 691-724, angles :20-30 and :61-109, circuit :452-496 with FRyz): the Y tree
+// of the real encoder over |v_i|, plus a phase tree over arg v_i (Z = left
+// phase - right phase, running phase = their mean). Each level's FRyz applies
+// Ry(Y) and then Rz(-Z) uniformly controlled by the higher qubits, so basis
+// state x ends with the product of cos / sin(Y/2) along its path and the phase
+// sum of +Z/2 (bit clear) / -Z/2 (bit set). One workgroup; LDS holds the
+// running magnitudes and phases and the Y and Z lists (4 x 2^q doubles).
+__global__ void __launch_bounds__(kBlock) kAEComplex(const double *__restrict__ vre, const double *__restrict__ vim,
+                                                     int q, double *__restrict__ ore, double *__restrict__ oim) {
+    extern __shared__ unsigned char smem[];
+    const int D = 1 << q;
+    double *mag = reinterpret_cast<double *>(smem), *ph = mag + D, *Y = ph + D, *Z = Y + D;
+    for (int i = threadIdx.x; i < D; i += kBlock) {
+        mag[i] = hypot(vre[i], vim[i]);
+        ph[i] = atan2(vim[i], vre[i]);
+    }
+    __syncthreads();
+    for (int l = 0; l < q; ++l) {
+        const int pairs = D >> (l + 1);
+        const int base = D - (D >> l);
+        for (int k = threadIdx.x; k < pairs; k += kBlock) {
+            const int left = k << (l + 1), right = left + (1 << l);
+            const double c = mag[left], s = mag[right];
+            Y[base + k] = 2.0 * atan2(s, c);
+            if (pairs != 1) {
+                const double m2 = c * c + s * s;
+                mag[left] = m2 > std::numeric_limits<double>::epsilon() ? sqrt(m2) : 0.0;
+            }
+            Z[base + k] = ph[left] - ph[right];
+            ph[left] = 0.5 * (ph[left] + ph[right]);
+        }
+        __syncthreads();
+    }
+    for (int x = threadIdx.x; x < D; x += kBlock) {
+        double amp = 1.0, phase = 0.0;
+        for (int l = 0; l < q; ++l) {
+            const int idx = D - (D >> l) + (x >> (l + 1));
+            const bool bit = (x >> l) & 1;
+            amp *= bit ? sin(0.5 * Y[idx]) : cos(0.5 * Y[idx]);
+            phase += bit ? -0.5 * Z[idx] : 0.5 * Z[idx];
+        }
+        ore[x] = amp * cos(phase);
+        oim[x] = amp * sin(phase);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) kAEWriteComplex(Amp<T> *__restrict__ a, const double *__restrict__ re,
+                                                          const double *__restrict__ im, int D) {
+    const int x = blockIdx.x * kBlock + threadIdx.x;
+    if (x < D) a[x] = Amp<T>{(T)re[x], (T)im[x]};
+}
 }  // namespace
+
+// qubits 0..q-1 take the encoded complex vector, the others stay |0>
+void amplitudeEncodeComplex(QuregDev *q, const double *re, const double *im, int qbits) {
+    const int D = 1 << qbits;
+    const size_t smem = (size_t)4 * D * sizeof(double);
+    if (smem > 65536 && !allowDynamicLds(reinterpret_cast<const void *>(kAEComplex), smem,
+                                         "qmxAmplitudeEncodeComplex"))
+        return;
+    double *buf = static_cast<double *>(devAlloc((size_t)4 * D * sizeof(double)));
+    hostToDevice(buf, re, (size_t)D * sizeof(double));
+    hostToDevice(buf + D, im, (size_t)D * sizeof(double));
+    hipLaunchKernelGGL(kAEComplex, dim3(1), dim3(kBlock), smem, hstream(), buf, buf + D, qbits, buf + 2 * D,
+                       buf + 3 * D);
+    initBlank(q);
+    QMX_DISPATCH(q->prec, T, {
+        hipLaunchKernelGGL(kAEWriteComplex<T>, dim3(gridFor(D, 1)), dim3(kBlock), 0, hstream(), (Amp<T> *)q->amps,
+                           buf + 2 * D, buf + 3 * D, D);
+    });
+    checkLaunch("qmxAmplitudeEncodeComplex");
+    devFree(buf);
+}
 
 // the reference's amplitude-encoding circuit (AmplitudeEncodeCuda.cu:112-150 angles,
 // Ry + uniformly controlled Ry ladder, FRGate.cu:181-252) applied to |0...0>:

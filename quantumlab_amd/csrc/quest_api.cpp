@@ -2187,6 +2187,98 @@ void qmxAmplitudeEncodeReal(Qureg qureg, const qreal *vec, int numQubits) {
     API_CATCH_VOID
 }
 
+// QuantumLab's complex-vector encoder (AmplitudeEncodeOneVector, bHasPhase
+// false: AmplitudeEncodeCuda.cu:691-724, ExchangeToYZGate :452-496, FRyz
+// FRGate.cu:400-460) as one state preparation: magnitude and phase trees, then
+// the product of each basis state's cos/sin factors and phase terms.
+void qmxAmplitudeEncodeComplex(Qureg qureg, const qreal *re, const qreal *im, int numQubits) {
+    API_TRY
+    qv::stateVecQureg(qureg, __func__);
+    if (numQubits < 1 || numQubits > 12 || numQubits > qureg.numQubitsRepresented)
+        qv::raise("Invalid number of qubits to encode. Must be >0, <=12 and <=numQubits.", __func__);
+    if (!re || !im) qv::raise("The vector is null.", __func__);
+    const size_t n = (size_t)1 << numQubits;
+    Doubles r(re, n), i(im, n);
+    qmx::amplitudeEncodeComplex(D(qureg), r.get(), i.get(), numQubits);
+    API_CATCH_VOID
+}
+
+namespace {
+// controls valid, distinct and apart from the target; 0 to 16 of them
+void uniformControls(const Qureg &q, const int *controls, int numControls, int target, const char *f) {
+    if (numControls < 0 || numControls > 16)
+        qv::raise("Invalid number of control qubits for a uniformly controlled gate. Must be >=0 and <=16.", f);
+    if (numControls == 0) {
+        qv::target(q, target, f);
+        return;
+    }
+    if (!controls) qv::raise("The control qubit list is null.", f);
+    qv::multiControlsTarget(q, controls, numControls, target, f);
+}
+
+// ops: 4 entries (row-major) per control value; density registers take the
+// conjugates on the column qubits
+void uniform2(const Qureg &q, const int *controls, int numControls, int target, const std::vector<Cplx> &ops) {
+    qmx::applyUniform2(D(q), controls, numControls, target, ops.data());
+    if (q.isDensityMatrix) {
+        const int s = q.numQubitsRepresented;
+        std::vector<int> cs(controls, controls + numControls);
+        for (int &c : cs) c += s;
+        std::vector<Cplx> oc(ops.size());
+        for (size_t e = 0; e < ops.size(); ++e) oc[e] = conjC(ops[e]);
+        qmx::applyUniform2(D(q), cs.data(), numControls, target + s, oc.data());
+    }
+}
+}  // namespace
+
+// A uniformly controlled (multiplexed) one-qubit gate: ops[k] on the target
+// where the controls read k. One sweep; QuantumLab composes these from 2^c
+// gates (FRGate.cu).
+void qmxUniformlyControlledUnitary(Qureg qureg, const int *controls, int numControls, int target,
+                                   const ComplexMatrix2 *ops) {
+    API_TRY
+    uniformControls(qureg, controls, numControls, target, __func__);
+    if (!ops) qv::raise("The matrix list is null.", __func__);
+    const size_t count = (size_t)1 << numControls;
+    std::vector<Cplx> m(4 * count);
+    for (size_t k = 0; k < count; ++k) {
+        qv::oneQubitUnitary(ops[k], __func__);
+        m2ToCplx(ops[k], &m[4 * k]);
+    }
+    uniform2(qureg, controls, numControls, target, m);
+    API_CATCH_VOID
+}
+
+// FRy / FRz / FRp (FRGate.cu:181-330, 460-515) in one sweep: Ry(angles[k]) as
+// rotateY, Rz(angles[k]) as rotateZ, or diag(1, e^{i angles[k]}) as phaseShift,
+// where the controls read k.
+void qmxUniformlyControlledRotation(Qureg qureg, const int *controls, int numControls, int target,
+                                    const qreal *angles, enum qmxUniformGate kind) {
+    API_TRY
+    uniformControls(qureg, controls, numControls, target, __func__);
+    if (kind != QMX_UNIFORM_RY && kind != QMX_UNIFORM_RZ && kind != QMX_UNIFORM_PHASE)
+        qv::raise("Invalid uniformly controlled gate kind. Must be QMX_UNIFORM_RY, QMX_UNIFORM_RZ or "
+                  "QMX_UNIFORM_PHASE.", __func__);
+    if (!angles) qv::raise("The angle list is null.", __func__);
+    const size_t count = (size_t)1 << numControls;
+    std::vector<Cplx> m(4 * count);
+    for (size_t k = 0; k < count; ++k) {
+        const double a = (double)angles[k];
+        Cplx *u = &m[4 * k];
+        if (kind == QMX_UNIFORM_RY) {
+            const double c = std::cos(a / 2), sn = std::sin(a / 2);
+            u[0] = Cplx{c, 0}, u[1] = Cplx{-sn, 0}, u[2] = Cplx{sn, 0}, u[3] = Cplx{c, 0};
+        } else if (kind == QMX_UNIFORM_RZ) {
+            u[0] = Cplx{std::cos(a / 2), -std::sin(a / 2)}, u[1] = Cplx{0, 0}, u[2] = Cplx{0, 0};
+            u[3] = Cplx{std::cos(a / 2), std::sin(a / 2)};
+        } else {
+            u[0] = Cplx{1, 0}, u[1] = Cplx{0, 0}, u[2] = Cplx{0, 0}, u[3] = Cplx{std::cos(a), std::sin(a)};
+        }
+    }
+    uniform2(qureg, controls, numControls, target, m);
+    API_CATCH_VOID
+}
+
 // Repeated measurement without re-uploading the state. QuantumLab's shot loop
 // (QLSimulatorMeasure.cpp:113-140) restores the register from the host before
 // every shot and calls measure() on each listed qubit in order, so shot s is a

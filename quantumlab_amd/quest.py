@@ -26,6 +26,7 @@ import types
 
 PAULI_I, PAULI_X, PAULI_Y, PAULI_Z = 0, 1, 2, 3
 QMX_PREC_FP64, QMX_PREC_FP32 = 0, 1
+QMX_UNIFORM_RY, QMX_UNIFORM_RZ, QMX_UNIFORM_PHASE = 2, 3, 4
 
 
 def _ints(xs: Iterable[int]):
@@ -277,6 +278,11 @@ def _make_abi(qreal) -> types.SimpleNamespace:
         "qmxDensityFusionStats": (None, [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "qmxSampleShots": (C.c_int, [Qureg, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
         "qmxAmplitudeEncodeReal": (None, [Qureg, C.POINTER(qreal), C.c_int]),
+        "qmxAmplitudeEncodeComplex": (None, [Qureg, C.POINTER(qreal), C.POINTER(qreal), C.c_int]),
+        "qmxUniformlyControlledUnitary": (None, [Qureg, C.POINTER(C.c_int), C.c_int, C.c_int,
+                                                 C.POINTER(ComplexMatrix2)]),
+        "qmxUniformlyControlledRotation": (None, [Qureg, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(qreal),
+                                                  C.c_int]),
         "qmxTimerBegin": (None, []),
         "qmxTimerEndMs": (C.c_double, []),
         "qmxTimerSlotBegin": (None, [C.c_int]),

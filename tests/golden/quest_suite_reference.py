@@ -6,15 +6,14 @@ rand() without seeding, so a given invocation is deterministic; a few of its
 sections fail on the reference build itself (a random 5-qubit matrix that
 misses the 1e-13 unitarity check, a 1e-13 comparison, distribution-limit
 checks whose rank count overflows int). tests/test_quest_suite.py runs the
-same tags the same way against this library and accepts exactly these.
+same tags the same way against this library and accepts these.
 
-Flavours: f64 (QuEST_PREC=2), f32 (QuEST_PREC=1) and f32_tol (QuEST_PREC=1
-with the suite's comparison tolerance widened to 1e-3, oracle/suite_tolerance.h).
+Flavours: f64 (QuEST_PREC=2) and f32 (QuEST_PREC=1).
 
 Run in the development container, where oracle/Makefile builds the suite
 against the reference sources (target questtests-ref); named flavours are
 re-run and merged into the existing file:
-    python tests/golden/quest_suite_reference.py [f64 f32 f32_tol]
+    python tests/golden/quest_suite_reference.py [f64 f32]
 """
 import json
 import os
@@ -30,7 +29,7 @@ TAGS = ["calculations", "data_structures", "decoherence", "gates", "operators", 
 
 def main():
     path = os.path.join(ROOT, "tests", "golden", "quest_suite_reference.json")
-    flavours = sys.argv[1:] or ["f64", "f32", "f32_tol"]
+    flavours = sys.argv[1:] or ["f64", "f32"]
     out = json.load(open(path))["results"] if os.path.exists(path) else {}
     for flavour in flavours:
         exe = os.path.join(ROOT, "oracle", "_ref", f"quest_tests_ref_{flavour}")
