@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(kBlock, 5) kFused5(Amp<float> *__restrict__ a,
 // streams. Each wave multiplies 16 groups out of LDS (the MFMA forms above) and
 // writes the results back in place; the workgroup then stores the tile. The
 // workgroups are persistent: the matrix fragments are read once into registers,
-// and the next tile's loads are in flight while the current one multiplies
+// and the next two tiles' loads are in flight while the current one multiplies
 // and is stored. LDS holds two tiles (double buffering, one barrier fewer).
 //
 // LDS slot of tile element e: a linear XOR swizzle (slotBit[i] = slot of
@@ -194,6 +194,9 @@ __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict_
     __shared__ __attribute__((aligned(16))) Amp<float> buf[2][kTileAmps];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, kq = lane >> 4;
+    // each workgroup walks a contiguous range of tiles (a grid-stride order,
+    // all workgroups on neighbouring tiles at once, measured 0.3-1.8 ms slower
+    // per block when targets are high or scattered)
     const long long first = xcdBlock() * sp.tilesPerGroup;
     const long long last = min(first + sp.tilesPerGroup, sp.numTiles);
     if (first >= last) return;  // the same for every thread of the workgroup
@@ -242,27 +245,23 @@ __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict_
     }
     const unsigned slot1 = sp.slotBit[0];  // slot(e + 1) = slot(e) ^ slot(1) for even e
 
-    f4v R[4];
-    uint64_t base = tileBase((uint64_t)first, sp);
+    // Two tiles' loads in flight: tile t + 2 is requested while tile t
+    // multiplies and is stored, into the register set tile t came in (two
+    // sets, the loop unrolled by two; LDS buffer = tile parity).
+    auto load = [&](f4v (&R)[4], long long t) {
+        const uint64_t base = tileBase((uint64_t)t, sp);
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-        R[j] = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(a + ((base | laneAddr) | jAddr[j])));
-    int b = 0;
-    for (long long t = first; t < last; ++t, b ^= 1) {
-        Amp<float> *T = buf[b];
+        for (int j = 0; j < 4; ++j)
+            R[j] = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(a + ((base | laneAddr) | jAddr[j])));
+    };
+    auto step = [&](f4v (&R)[4], long long t, Amp<float> *T) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             T[laneSlot ^ jSlot[j]] = Amp<float>{R[j].x, R[j].y};
             T[laneSlot ^ jSlot[j] ^ slot1] = Amp<float>{R[j].z, R[j].w};
         }
         __syncthreads();
-        const uint64_t cur = base;
-        if (t + 1 < last) {
-            base = tileBase((uint64_t)(t + 1), sp);
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                R[j] = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(a + ((base | laneAddr) | jAddr[j])));
-        }
+        if (t + 2 < last) load(R, t + 2);
         // both row tiles accumulate together, so an operand is live for one step
         f4 acc[2][3];
 #pragma unroll
@@ -288,12 +287,20 @@ __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict_
                 T[outLane ^ sOut[4 * h + v]] = Amp<float>{acc[h][0][v] - acc[h][1][v],
                                                           acc[h][2][v] - acc[h][0][v] - acc[h][1][v]};
         __syncthreads();
+        const uint64_t cur = tileBase((uint64_t)t, sp);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const Amp<float> lo = T[laneSlot ^ jSlot[j]], hi = T[laneSlot ^ jSlot[j] ^ slot1];
             __builtin_nontemporal_store(f4v{lo.x, lo.y, hi.x, hi.y},
                                         reinterpret_cast<f4v *>(a + ((cur | laneAddr) | jAddr[j])));
         }
+    };
+    f4v R0[4], R1[4];
+    load(R0, first);
+    if (first + 1 < last) load(R1, first + 1);
+    for (long long t = first; t < last; t += 2) {
+        step(R0, t, buf[0]);
+        if (t + 1 < last) step(R1, t + 1, buf[1]);
     }
 }
 
