@@ -55,6 +55,22 @@ FP32_MATRIX_PEAK_TFLOPS = 157.3
 # FP32 FMA on the VALU without packed instructions: half of the published
 # 157.3 TFLOP/s vector peak, which counts packed FP32 (not used here)
 FP32_VALU_PEAK_TFLOPS = 78.6
+
+
+def profiled_traffic(name, kernel, qubits=None):
+    """HBM bytes per launch of `kernel` (name prefix) from the calibrated
+    FETCH_SIZE / WRITE_SIZE summary profiles/<name>_traffic.json that
+    quantumlab_amd.profile_summary wrote for the same workload; None without it."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    tj = json.load(open(path))
+    if qubits is not None and tj.get("qubits") not in (None, qubits):
+        return None, None
+    for key, b in sorted(tj.get("bytes_per_dispatch", {}).items()):
+        if key.split("|")[0].startswith(kernel):
+            return b, f"profiles/{name}_pmc.md"
+    return None, None
 PERIOD = 5
 
 
@@ -168,9 +184,9 @@ def run_fp32_large(qe, n, seed, k):
             "fp64_per_gate_gb_per_s": round(W.gate_bytes(gates, n, 16) / wall64 / 1e9, 1),
             "speedup_vs_fp64_per_gate": round(wall64 / wall, 3),
             "fidelity_error": float(1 - fid), "max_abs_amp_error": float(maxabs),
-            "roofline": {"bound": "hbm", "kernel": "kFused5", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+            "roofline": {"bound": "hbm", "kernel": "kFused5Tile", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                         "mfma_tflops": round(blocks * N * 256 / wall / 1e12, 2)}}
+                         "mfma_tflops": round(blocks * N * 192 / wall / 1e12, 2)}}
 
 
 def host_cpu():
@@ -312,7 +328,9 @@ def run_density(qe, n, layers, seed):
     res.update(outs)
     res["roofline"] = {"bound": "hbm", "kernel": "kDensePairMfma<double> (two composed 2-qubit superoperators per sweep of the 2n-qubit vector, on the matrix cores)",
                        "achieved": a["gb_per_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(a["gb_per_s"] / HBM_PEAK_GBS, 4), "traffic": None,
+                       "frac": round(a["gb_per_s"] / HBM_PEAK_GBS, 4),
+                       "traffic": profiled_traffic("density14_fp64", "kDensePairMfma<double>", n)[0],
+                       "traffic_source": profiled_traffic("density14_fp64", "kDensePairMfma<double>", n)[1],
                        "note": "read + write of every entry per sweep (2 * 4^n * 16 B) over the leg's wall time; "
                                "all operations compose, so the sweeps are the only device traffic; a sweep applies "
                                "two 2-qubit superoperators (32 complex FMAs per entry). "
@@ -434,7 +452,9 @@ def run_trotter(qe, n, count, seed, eps=1e-4, sd_count=16):
             "pauli_exponentials_per_s_learned": round(exps / tl_all.sum(), 1),
             "roofline": {"bound": "hbm", "kernel": "kPauliCoset<float> (fused Pauli-rotation sweeps)",
                          "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                         "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "traffic": profiled_traffic("trotter24_fp32", "kPauliCoset<float>", n)[0],
+                         "traffic_source": profiled_traffic("trotter24_fp32", "kPauliCoset<float>", n)[1],
                          "sweeps": int(sum(x[6] for x in rows)), "bytes_per_sweep": 2.0 * (1 << n) * 8,
                          "note": "read + write of the FP32 register per fused block, over the learned solves' wall "
                                  "time, host planning and launch gaps included; commuting rotations are packed into "
@@ -550,7 +570,9 @@ def run_swap_tests(qe, points, centroids, iterations, warmup, seed):
            "roofline": {"bound": "valu", "kernel": "kSwapTilesMfma (32x32x2 FP32 MFMA blocks of the amplitude matrix, "
                                                    "squares summed on the VALU)",
                         "achieved": round(valu_tflops, 2), "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": round(valu_tflops / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                        "frac": round(valu_tflops / FP32_VALU_PEAK_TFLOPS, 4),
+                        "traffic": profiled_traffic("qkmeans_swap", "kSwapTilesMfma")[0],
+                        "traffic_source": profiled_traffic("qkmeans_swap", "kSwapTilesMfma")[1],
                         "avg_launch_ms": round(tile_ms, 4), "mfma_blocks_per_launch": blocks,
                         "valu_flops_per_block": 2048, "mfma_tflops": round(tflops, 2),
                         "mfma_frac": round(tflops / FP32_MATRIX_PEAK_TFLOPS, 4),
@@ -668,13 +690,7 @@ def statevec_leg(qe, args, n, N):
     tmeans = {t: float(np.mean(v)) for t, v in per_target.items()}
     # HBM bytes per launch of the dominant kernel, from the calibrated PMC runs
     # summarised in profiles/brick32_fp64_pmc_traffic.md (same workload)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "brick32_fp64_traffic.json")
-    if os.path.exists(tpath):
-        tj = json.load(open(tpath))
-        key = f"kMatrix2<double, false, true>|{N // 2}"
-        if tj.get("qubits") == n and key in tj["bytes_per_dispatch"]:
-            traffic = tj["bytes_per_dispatch"][key]
+    traffic, traffic_src = profiled_traffic("brick32_fp64", "kMatrix2<double, false, true>", n)
 
     result = {
         "metric": METRIC,
@@ -699,7 +715,7 @@ def statevec_leg(qe, args, n, N):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(float(cu_ms.mean()), 4), "launches": int(len(cu_ms)),
                      "algorithmic_bytes_per_launch": cu_bytes,
-                     "traffic_source": "profiles/brick32_fp64_pmc_traffic.md" if traffic else None,
+                     "traffic_source": traffic_src,
                      "per_target_ms_min": round(min(tmeans.values()), 4),
                      "per_target_ms_max": round(max(tmeans.values()), 4)},
         "event_time_s": round(ev_ms / 1e3, 4),
@@ -801,11 +817,15 @@ def main():
                                 "max_prob_error": float(p_err), "norm2_before_renorm": float(norm2)}
         fgbs = blocks * 2.0 * N * 8 / wall32 / 1e9
         result["fused_fp32"]["roofline"] = {
-            "bound": "hbm", "kernel": "kFused5 (5-qubit blocks, v_mfma_f32_16x16x4f32)", "achieved": round(fgbs, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(fgbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "bound": "hbm", "kernel": "kFused5Tile (5-qubit blocks on LDS tiles, v_mfma_f32_16x16x4f32, "
+                                      "three real products per complex product)", "achieved": round(fgbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(fgbs / HBM_PEAK_GBS, 4),
+            "traffic": profiled_traffic("fused32_fp32", "kFused5Tile", n)[0],
+            "traffic_source": profiled_traffic("fused32_fp32", "kFused5Tile", n)[1],
             "bytes_per_block": 2.0 * N * 8,
-            "mfma_tflops": round(blocks * N * 256 / wall32 / 1e12, 2),
-            "note": "every block reads and writes the FP32 register once; 256 MFMA flops per amplitude"}
+            "mfma_tflops": round(blocks * N * 192 / wall32 / 1e12, 2),
+            "note": "every block reads and writes the FP32 register once; 192 MFMA flops per amplitude; over the "
+                    "fused steps' wall time (launch gaps included)"}
         log(f"fused fp32: {wall / wall32:.2f}x vs fp64 per-gate, 1-F={1 - fid:.3e}, max|da|={maxabs:.3e}")
         qe.destroy(q32)
     if q64 is not None:

@@ -78,6 +78,7 @@ def main():
     ap.add_argument("--sq", default=None)
     ap.add_argument("--factors", default=None, help="a _traffic.json with fetch/write factors (calibration)")
     ap.add_argument("--calibration-qubits", type=int, default=32)
+    ap.add_argument("--qubits", type=int, default=None, help="the workload's register size (recorded)")
     ap.add_argument("--out", default="profiles")
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
@@ -88,7 +89,7 @@ def main():
     if not (args.fetch or args.sq):
         return
     lines = [f"# {args.name}: counters per dispatch (rocprofv3 PMC)", "", "Commands:"]
-    summary = {"name": args.name, "bench_args": args.bench_args}
+    summary = {"name": args.name, "bench_args": args.bench_args, "qubits": args.qubits}
     if args.fetch and args.write:
         fetch, calls = counters(args.fetch)
         write, _ = counters(args.write)
@@ -151,7 +152,7 @@ def main():
     cmds.append(f"python3 -m quantumlab_amd.profile_summary --name {args.name} --bench-args \"{args.bench_args}\" "
                 f"--trace {args.trace}" + (f" --fetch {args.fetch} --write {args.write}" if args.fetch else "")
                 + (f" --sq {args.sq}" if args.sq else "") + (f" --factors {args.factors}" if args.factors else "")
-                + f" --out {args.out}")
+                + (f" --qubits {args.qubits}" if args.qubits else "") + f" --out {args.out}")
     lines += [f"- `{c}`" for c in cmds] + [""] + traffic_text + sq_text
     with open(os.path.join(args.out, f"{args.name}_pmc.md"), "w") as f:
         f.write("\n".join(lines) + "\n")

@@ -562,3 +562,28 @@ def test_prob_of_all_outcomes_matches_oracle(gpu, cpuref, density, n):
         assert np.max(np.abs(a - b)) < 1e-12 * scale, qs
     for lib, q in zip((gpu, cpuref), regs):
         lib.destroy(q)
+
+
+@pytest.mark.parametrize("n,targets", [(11, [0, 1, 2, 3, 4]), (11, [6, 7, 8, 9, 10]), (12, [0, 3, 7, 10, 11]),
+                                       (13, [4, 0, 12, 2, 1]), (14, [5, 6, 7, 8, 9]), (14, [13, 0, 9, 1, 12]),
+                                       (10, [9, 0, 4, 2, 7]), (9, [0, 1, 2, 3, 4])])
+def test_fp32_five_target_blocks_every_placement(gpu, n, targets):
+    """Uncontrolled 5-target multiQubitUnitary on FP32 registers (the fused
+    MFMA kernels: LDS tiles from 11 qubits, the direct form below) against the
+    FP64 dense product, for low, high, mid, scattered and unsorted targets."""
+    rng = np.random.default_rng(n + sum(targets))
+    psi = random_state(n, 3 + n)
+    q = gpu.qureg(n, precision=QMX_PREC_FP32)
+    gpu.set_state(q, psi)
+    want = psi.astype(complex)
+    idx = np.arange(1 << n)
+    for _ in range(3):
+        u = W.haar_unitary(rng, 32)
+        gpu.multiQubitUnitary(q, targets, u)
+        mask = sum(1 << t for t in targets)
+        rest = idx[(idx & mask) == 0]
+        offs = np.array([sum(((c >> j) & 1) << targets[j] for j in range(5)) for c in range(32)])
+        X = want[rest[:, None] + offs[None, :]]
+        want[rest[:, None] + offs[None, :]] = X @ u.T
+    assert np.max(np.abs(gpu.state(q) - want)) < 2e-6
+    gpu.destroy(q)
