@@ -14,10 +14,13 @@ per build and tag (tests/golden/quest_suite_reference.py). The suite draws its
 random states and matrices from rand() without a seed, so two builds that fail
 the same sections also draw the same inputs; after a section ends early the
 rand() stream shifts, and which random matrix misses the unitarity or
-trace-preservation check (1e-5 in float) moves with it. Accepted here:
-  f64  only sections QuEST's own build fails;
-  f32  those, plus sections failing only by a validation error on a matrix the
-       suite drew at random, or only by tolerance comparisons (areEqual /
+trace-preservation check (1e-13 in double, 1e-5 in float) moves with it; the
+suite's own random 4- and 5-qubit unitaries (utilities.cpp:558-600, accepted
+by it at 1e-11) miss the 1e-13 check now and then, on QuEST's build as here.
+Accepted here:
+  f64  sections QuEST's own build fails, plus sections failing only by a
+       validation error on a matrix the suite drew at random;
+  f32  those, plus sections failing only by tolerance comparisons (areEqual /
        Approx at 1e-5: the suite's 5-qubit density-matrix debug states hold
        entries up to 204.7, where one float ulp is 1.5e-5, so such a comparison
        passes only when the library rounds in the order of the suite's own
@@ -77,7 +80,7 @@ def is_random_input_error(block):
     return head.startswith("due to unexpected exception") and any(e in head for e in RANDOM_INPUT_ERRORS)
 
 
-ACCEPTED_KINDS = {"f64": (), "f32": (is_random_input_error, is_comparison)}
+ACCEPTED_KINDS = {"f64": (is_random_input_error,), "f32": (is_random_input_error, is_comparison)}
 
 
 def unexplained_failures(flavour, failed, ref, out):
@@ -139,7 +142,8 @@ def test_failure_report_parser():
     assert not is_comparison(kraus) and not is_random_input_error(kraus)
     assert failed_sections(out) == set(b)
     failed = set(b)
-    assert set(unexplained_failures("f64", failed, set(), out)) == failed
+    assert set(unexplained_failures("f64", failed, set(), out)) == {"test_unitaries.cpp:565",
+                                                                    "test_decoherence.cpp:1090"}
     assert set(unexplained_failures("f32", failed, set(), out)) == {"test_decoherence.cpp:1090"}
     assert not unexplained_failures("f64", failed, failed, out)
 
