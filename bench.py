@@ -189,6 +189,11 @@ def run_fp32_large(qe, n, seed, k):
                          "mfma_tflops": round(blocks * N * 192 / wall / 1e12, 2)}}
 
 
+# CPUs this process may use, read before any OpenMP runtime starts (with
+# OMP_PROC_BIND set it pins the calling thread to its first place)
+CPUS_ALLOWED = len(os.sched_getaffinity(0))
+
+
 def host_cpu():
     """The host the CPU baselines run on: model, physical cores, logical CPUs,
     the CPUs this process may use, and the OpenMP settings in force."""
@@ -210,7 +215,7 @@ def host_cpu():
     except OSError:
         pass
     return {"model": model, "physical_cores": len(cores) or None, "logical_cpus": os.cpu_count(),
-            "cpus_allowed": len(os.sched_getaffinity(0)), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "cpus_allowed": CPUS_ALLOWED, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
             "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES")}
 
 
