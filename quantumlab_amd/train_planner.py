@@ -193,6 +193,48 @@ def fit(qe, X, L, sizes, Xc, Lc, sizes_c, epochs, seed, quantile, lr=2e-3, batch
                          "train_loss_last": float(losses[-1])}
 
 
+def load_calibration_cache(args):
+    """(labels, done) for at least calib_count instances of the calibration
+    seed and size: the cache's entries where it matches, NaN / False after."""
+    n = args.calib_count
+    labels, done = np.full((n, 2), np.nan), np.zeros(n, dtype=bool)
+    if args.calib_labels and os.path.exists(args.calib_labels):
+        z = np.load(args.calib_labels, allow_pickle=False)
+        if (int(z["seed"]) == args.seed + 1 and int(z["size"]) == args.calib_size
+                and str(z["method"]) == "bisection"):
+            zl = np.asarray(z["labels"], dtype=np.float64)
+            zd = np.asarray(z["done"], dtype=bool) if "done" in z.files else np.ones(len(zl), dtype=bool)
+            if len(zl) > n:
+                labels, done = np.full((len(zl), 2), np.nan), np.zeros(len(zl), dtype=bool)
+            labels[:len(zl)], done[:len(zl)] = zl, zd
+    return labels, done
+
+
+def save_calibration_cache(args, labels, done):
+    os.makedirs(os.path.dirname(os.path.abspath(args.calib_labels)), exist_ok=True)
+    np.savez(args.calib_labels, labels=labels, done=done, seed=args.seed + 1, size=args.calib_size,
+             method="bisection")
+
+
+def label_calibration_range(args, a, b):
+    """Label calibration instances a..b-1 (those not yet in the cache), saving
+    after each one."""
+    import quantumlab_amd
+    qe = quantumlab_amd.load()
+    args.calib_count = max(args.calib_count, b)
+    labels, done = load_calibration_cache(args)
+    t0 = time.perf_counter()
+    for i in range(a, b):
+        if done[i]:
+            continue
+        inst = H.trotter_instance(args.calib_size, args.seed + 1, i)
+        labels[i] = label_large(qe, [inst])[0] if args.calib_size > 12 else label(qe, [inst])[0]
+        done[i] = True
+        save_calibration_cache(args, labels, done)
+        log(f"calibration instance {i}: log2 r* = {labels[i]}; {time.perf_counter() - t0:.1f} s")
+    log(f"calibration labels done: {int(done.sum())} of {len(done)}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--count", type=int, default=20000)
@@ -208,8 +250,16 @@ def main():
     ap.add_argument("--summary", type=str, default=None)
     ap.add_argument("--calib-labels", type=str,
                     default=os.path.join(os.path.dirname(__file__), "data", "trotter_calib_labels.npz"),
-                    help="cache of the calibration labels (reused when its seed, size and count match)")
+                    help="cache of the calibration labels (reused when its seed and size match; missing entries "
+                         "are labelled and added)")
+    ap.add_argument("--calib-range", type=str, default=None,
+                    help="A:B — only label calibration instances A..B-1 into the cache (saved after each "
+                         "instance, so the work can be split over several runs), then stop")
     args = ap.parse_args()
+    if args.calib_range:
+        a, b = (int(v) for v in args.calib_range.split(":"))
+        label_calibration_range(args, a, b)
+        return
     import quantumlab_amd
     qe = quantumlab_amd.load()
     sizes = [int(s) for s in args.sizes.split(",")]
@@ -225,18 +275,16 @@ def main():
     ns = np.array([i.n for i in insts])
     log(f"small-register labels in {t_label:.1f} s; labelled fractions {np.isfinite(L).mean(0)}")
     t1 = time.perf_counter()
-    Lc = None
-    if args.calib_labels and os.path.exists(args.calib_labels):
-        z = np.load(args.calib_labels, allow_pickle=False)
-        if (int(z["seed"]) == args.seed + 1 and int(z["size"]) == args.calib_size and len(z["labels"]) == len(calib)
-                and str(z["method"]) == "bisection"):
-            Lc = np.asarray(z["labels"], dtype=np.float64)
-            log(f"calibration labels from {args.calib_labels}")
-    if Lc is None:
-        Lc = label(qe, calib) if args.calib_size <= 12 else label_large(qe, calib)
-        if args.calib_labels:
-            os.makedirs(os.path.dirname(os.path.abspath(args.calib_labels)), exist_ok=True)
-            np.savez(args.calib_labels, labels=Lc, seed=args.seed + 1, size=args.calib_size, method="bisection")
+    labels, done = load_calibration_cache(args)
+    missing = [i for i in range(len(calib)) if not done[i]]
+    if missing:
+        Lm = label(qe, [calib[i] for i in missing]) if args.calib_size <= 12 else \
+            label_large(qe, [calib[i] for i in missing])
+        labels[missing], done[missing] = Lm, True
+        save_calibration_cache(args, labels, done)
+    else:
+        log(f"calibration labels from {args.calib_labels}")
+    Lc = labels[:len(calib)]
     t_calib = time.perf_counter() - t1
     log(f"{args.calib_size}-qubit labels in {t_calib:.1f} s; labelled fractions {np.isfinite(Lc).mean(0)}")
     for n in sizes:
