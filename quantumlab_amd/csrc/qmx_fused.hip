@@ -144,10 +144,19 @@ __global__ void __launch_bounds__(kBlock, 5) kFused5(Amp<float> *__restrict__ a,
 // runs of at least 64 amplitudes (512 B): it moves between HBM and LDS as
 // coalesced nontemporal 16 B accesses whatever the targets, like the in-place
 // streams. Each wave multiplies 16 groups out of LDS (the MFMA forms above) and
-// writes the results back in place; the workgroup then stores the tile. The
-// workgroups are persistent: the matrix fragments are read once into registers,
-// and the next two tiles' loads are in flight while the current one multiplies
-// and is stored. LDS holds two tiles (double buffering, one barrier fewer).
+// writes the results back in place; the workgroup then stores the tile. LDS
+// holds two tiles (double buffering, one barrier fewer), and the next two
+// tiles' loads are in flight while the current one multiplies and is stored.
+//
+// Each workgroup takes a few consecutive tiles (kTilesPerGroup), and the
+// hardware dispatches workgroups in index order, so at any moment the chip
+// works on a compact window of the register. A persistent grid (one long
+// range of tiles per workgroup, 768 ranges streamed at once) measured 7-8 %
+// slower at 32 qubits, with or without the products: the pure load/store
+// skeleton of the tile pattern took 12.9 ms per 32 GiB sweep persistent and
+// 11.9 ms with one tile per workgroup. What a workgroup needs besides its
+// tiles comes precomputed on the host (the tile form below), so its prologue
+// is 13 coalesced loads.
 //
 // LDS slot of tile element e: a linear XOR swizzle (slotBit[i] = slot of
 // element 2^i) chosen on the host so that the 16 lanes of an MFMA operand read
@@ -156,16 +165,20 @@ __global__ void __launch_bounds__(kBlock, 5) kFused5(Amp<float> *__restrict__ a,
 constexpr int kTileLog = 11;
 constexpr int kTileGroups = 64;
 constexpr int kTileAmps = 32 * kTileGroups;
+constexpr int kTilesPerGroup = 4;
 
 struct Fused5TileSpec {
     int pos[kTileLog];          // P, ascending
     int tIdx[5];                // index in P of the j-th lowest target (combination bit j)
     int gIdx[6];                // index in P of group bit b
     unsigned slotBit[kTileLog];  // swizzle: slot of element 2^i
+    // workgroup-uniform slots and offsets (see kFused5Tile)
+    unsigned sIn[8], sOut[8], jSlot[4];
+    uint64_t jAddr[4];
     long long numTiles, tilesPerGroup;
 };
 
-__device__ __forceinline__ unsigned tileSlot(unsigned e, const Fused5TileSpec &sp) {
+__host__ __device__ __forceinline__ unsigned tileSlot(unsigned e, const Fused5TileSpec &sp) {
     unsigned s = 0;
 #pragma unroll
     for (int i = 0; i < kTileLog; ++i)
@@ -173,7 +186,7 @@ __device__ __forceinline__ unsigned tileSlot(unsigned e, const Fused5TileSpec &s
     return s;
 }
 
-__device__ __forceinline__ unsigned tileCombo(unsigned c, const Fused5TileSpec &sp) {
+__host__ __device__ __forceinline__ unsigned tileCombo(unsigned c, const Fused5TileSpec &sp) {
     unsigned e = 0;
 #pragma unroll
     for (int j = 0; j < 5; ++j) e |= ((c >> j) & 1u) << sp.tIdx[j];
@@ -187,62 +200,40 @@ __device__ __forceinline__ uint64_t tileBase(uint64_t t, const Fused5TileSpec &s
     return t;
 }
 
+// The tile form of a block, built on the host (kFused5TileFormBytes, 16 KiB):
+//   frag[i][lane] (i < 12, float4): lane (col = lane % 16, kq = lane / 16)'s
+//     48 matrix fragments M_p[16 h + col][4 st + kq], fragment f = (3 h + p) 8 +
+//     st in component f % 4 of vector f / 4 (one coalesced 1 KiB load each);
+//   lanes[tid] (uint4): {inLane | outLane << 16, laneSlot, laneAddr low, high}
+//     for thread tid of a workgroup (LDS slots of its MFMA operands and of its
+//     transfer elements, and the state offset of its transfer elements).
+constexpr int kFragVecs = 12;
+constexpr size_t kFused5TileFormBytes = kFragVecs * 64 * 16 + kBlock * 16;
+
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict__ a, Fused5TileSpec sp,
-                                                      const float *__restrict__ M) {
+                                                      const f4v *__restrict__ form) {
     __shared__ __attribute__((aligned(16))) Amp<float> buf[2][kTileAmps];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int col = lane & 15, kq = lane >> 4;
-    // each workgroup walks a contiguous range of tiles (a grid-stride order,
-    // all workgroups on neighbouring tiles at once, measured 0.3-1.8 ms slower
-    // per block when targets are high or scattered)
+    const int tid = threadIdx.x, lane = tid & 63;
     const long long first = xcdBlock() * sp.tilesPerGroup;
     const long long last = min(first + sp.tilesPerGroup, sp.numTiles);
     if (first >= last) return;  // the same for every thread of the workgroup
 
-    // matrix fragments, once: product p, row tile h, step s -> M_p[16h + col][4s + kq]
-    float fr[2][3][8];
+    // matrix fragments: product p, row tile h, step st -> M_p[16h + col][4st + kq]
+    f4v fv[kFragVecs];
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int p = 0; p < 3; ++p)
-#pragma unroll
-            for (int st = 0; st < 8; ++st) fr[h][p][st] = M[p * 1024 + (16 * h + col) * 32 + 4 * st + kq];
-
-    // LDS slots of this lane's operands: group g = 16 wave + col; input
-    // combination 4s + kq, output combination 16h + 4kq + v (linear swizzle:
-    // slot(x | y) = slot(x) ^ slot(y) for disjoint bits)
-    const unsigned g = 16 * wave + col;
-    unsigned eG = 0;
-#pragma unroll
-    for (int b = 0; b < 6; ++b) eG |= ((g >> b) & 1u) << sp.gIdx[b];
-    const unsigned inLane = tileSlot(eG | tileCombo(kq, sp), sp);
-    const unsigned outLane = tileSlot(eG | tileCombo(4 * kq, sp), sp);
-    unsigned sIn[8], sOut[8];
-#pragma unroll
-    for (int st = 0; st < 8; ++st) sIn[st] = tileSlot(tileCombo(4 * st, sp), sp);
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) sOut[4 * h + v] = tileSlot(tileCombo(16 * h + v, sp), sp);
-
-    // coalesced transfers: thread tid moves elements e = 2 tid + 512 j and e + 1
-    // (j < 4); bits 0-5 of e are qubits 0-5, bits 6-10 go to P[6..10]. The
-    // parts from tid (bits 0-8) and from j (bits 9-10) are disjoint, so address
-    // and slot split into a per-lane and a uniform term.
-    const unsigned eLane = 2 * tid;
-    uint64_t laneAddr = eLane & 63u;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) laneAddr |= (uint64_t)((eLane >> (6 + i)) & 1u) << sp.pos[6 + i];
-    const unsigned laneSlot = tileSlot(eLane, sp);
-    uint64_t jAddr[4];
-    unsigned jSlot[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        jAddr[j] = ((uint64_t)(j & 1) << sp.pos[9]) | ((uint64_t)(j >> 1) << sp.pos[10]);
-        jSlot[j] = tileSlot(512u * j, sp);
-    }
+    for (int i = 0; i < kFragVecs; ++i) fv[i] = form[i * 64 + lane];
+    const uint4 L = reinterpret_cast<const uint4 *>(form + kFragVecs * 64)[tid];
+    // LDS slots of this lane's operands (group g = 16 wave + col; input
+    // combination 4s + kq, output combination 16h + 4kq + v; linear swizzle:
+    // slot(x | y) = slot(x) ^ slot(y) for disjoint bits) and of its transfer
+    // elements e = 2 tid + 512 j and e + 1 (j < 4): bits 0-5 of e are qubits
+    // 0-5, bits 6-10 go to P[6..10]; the parts from tid (bits 0-8) and from j
+    // (bits 9-10) are disjoint, so address and slot split into a per-lane and
+    // a uniform term.
+    const unsigned inLane = L.x & 0xffffu, outLane = L.x >> 16, laneSlot = L.y;
+    const uint64_t laneAddr = (uint64_t)L.z | ((uint64_t)L.w << 32);
     const unsigned slot1 = sp.slotBit[0];  // slot(e + 1) = slot(e) ^ slot(1) for even e
 
     // Two tiles' loads in flight: tile t + 2 is requested while tile t
@@ -252,13 +243,13 @@ __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict_
         const uint64_t base = tileBase((uint64_t)t, sp);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            R[j] = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(a + ((base | laneAddr) | jAddr[j])));
+            R[j] = __builtin_nontemporal_load(reinterpret_cast<const f4v *>(a + ((base | laneAddr) | sp.jAddr[j])));
     };
     auto step = [&](f4v (&R)[4], long long t, Amp<float> *T) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            T[laneSlot ^ jSlot[j]] = Amp<float>{R[j].x, R[j].y};
-            T[laneSlot ^ jSlot[j] ^ slot1] = Amp<float>{R[j].z, R[j].w};
+            T[laneSlot ^ sp.jSlot[j]] = Amp<float>{R[j].x, R[j].y};
+            T[laneSlot ^ sp.jSlot[j] ^ slot1] = Amp<float>{R[j].z, R[j].w};
         }
         __syncthreads();
         if (t + 2 < last) load(R, t + 2);
@@ -270,13 +261,14 @@ __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict_
             for (int p = 0; p < 3; ++p) acc[h][p] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
-            const Amp<float> x = T[inLane ^ sIn[st]];
+            const Amp<float> x = T[inLane ^ sp.sIn[st]];
             const float xs = x.x + x.y;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                acc[h][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr[h][0][st], x.x, acc[h][0], 0, 0, 0);
-                acc[h][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr[h][1][st], x.y, acc[h][1], 0, 0, 0);
-                acc[h][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(fr[h][2][st], xs, acc[h][2], 0, 0, 0);
+                const int f0 = (3 * h) * 8 + st, f1 = (3 * h + 1) * 8 + st, f2 = (3 * h + 2) * 8 + st;
+                acc[h][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(fv[f0 >> 2][f0 & 3], x.x, acc[h][0], 0, 0, 0);
+                acc[h][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(fv[f1 >> 2][f1 & 3], x.y, acc[h][1], 0, 0, 0);
+                acc[h][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(fv[f2 >> 2][f2 & 3], xs, acc[h][2], 0, 0, 0);
             }
         }
         // the wave's reads of these 16 groups are done: results go back in place
@@ -284,15 +276,15 @@ __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict_
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                T[outLane ^ sOut[4 * h + v]] = Amp<float>{acc[h][0][v] - acc[h][1][v],
-                                                          acc[h][2][v] - acc[h][0][v] - acc[h][1][v]};
+                T[outLane ^ sp.sOut[4 * h + v]] = Amp<float>{acc[h][0][v] - acc[h][1][v],
+                                                             acc[h][2][v] - acc[h][0][v] - acc[h][1][v]};
         __syncthreads();
         const uint64_t cur = tileBase((uint64_t)t, sp);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const Amp<float> lo = T[laneSlot ^ jSlot[j]], hi = T[laneSlot ^ jSlot[j] ^ slot1];
+            const Amp<float> lo = T[laneSlot ^ sp.jSlot[j]], hi = T[laneSlot ^ sp.jSlot[j] ^ slot1];
             __builtin_nontemporal_store(f4v{lo.x, lo.y, hi.x, hi.y},
-                                        reinterpret_cast<f4v *>(a + ((cur | laneAddr) | jAddr[j])));
+                                        reinterpret_cast<f4v *>(a + ((cur | laneAddr) | sp.jAddr[j])));
         }
     };
     f4v R0[4], R1[4];
@@ -306,7 +298,8 @@ __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict_
 
 struct Fused5Prepared {
     Fused5Spec spec;
-    float *M;  // device copy of the block's three real 32x32 matrices (Ur, Ui, Ur + Ui)
+    float *M;     // device copy of the block's three real 32x32 matrices (Ur, Ui, Ur + Ui)
+    void *form;   // device copy of its tile form
 };
 
 constexpr int kFused5MatrixFloats = 3 * 32 * 32;
@@ -343,9 +336,9 @@ void fused5Build(const int *targs, const Cplx *u, Fused5Spec &spec, float *hm) {
         }
 }
 
-// The tile form's index positions and LDS swizzle for a block whose ascending
-// targets are t[0..4].
-Fused5TileSpec tileSpecOf(const BitList &targets, long long numAmps) {
+// The tile form's index positions, LDS swizzle and workgroup-uniform slots and
+// offsets for a block whose ascending targets are t[0..4].
+Fused5TileSpec tileSpecOf(const BitList &targets) {
     Fused5TileSpec sp{};
     uint64_t targ = 0;
     for (int j = 0; j < 5; ++j) targ |= 1ull << targets.pos[j];
@@ -377,28 +370,57 @@ Fused5TileSpec tileSpecOf(const BitList &targets, long long numAmps) {
         used[f] = true;
         sp.slotBit[sp.gIdx[b]] |= 1u << f;
     }
-    sp.numTiles = numAmps / kTileAmps;
+    for (int st = 0; st < 8; ++st) sp.sIn[st] = tileSlot(tileCombo(4 * st, sp), sp);
+    for (int h = 0; h < 2; ++h)
+        for (int v = 0; v < 4; ++v) sp.sOut[4 * h + v] = tileSlot(tileCombo(16 * h + v, sp), sp);
+    for (int j = 0; j < 4; ++j) {
+        sp.jAddr[j] = ((uint64_t)(j & 1) << sp.pos[9]) | ((uint64_t)(j >> 1) << sp.pos[10]);
+        sp.jSlot[j] = tileSlot(512u * j, sp);
+    }
     return sp;
 }
 
-void launchFused5(QuregDev *q, const Fused5Spec &spec, const float *M) {
-    if (q->numQubits >= kTileLog) {
-        Fused5TileSpec sp = tileSpecOf(spec.targetsSorted, q->numAmps);
-        // persistent workgroups: 3 per CU (the kernel's occupancy), a multiple of 8
-        // (one share per XCD)
-        static long long grid = 0;
-        if (!grid) {
-            int cus = 256;
-            hipDeviceProp_t prop;
-            if (hipGetDeviceProperties(&prop, 0) == hipSuccess) cus = prop.multiProcessorCount;
-            grid = std::max(8, (cus * 3) / 8 * 8);
+// The tile form (layout at kFused5Tile) of a block from its row-major matrices
+// hm (fused5Build) and its tile spec.
+void tileFormBuild(const float *hm, const Fused5TileSpec &sp, void *out) {
+    float *frag = static_cast<float *>(out);
+    for (int lane = 0; lane < 64; ++lane) {
+        const int col = lane & 15, kq = lane >> 4;
+        for (int f = 0; f < 4 * kFragVecs; ++f) {
+            const int h = f / 24, p = (f / 8) % 3, st = f % 8;
+            frag[((f >> 2) * 64 + lane) * 4 + (f & 3)] = hm[p * 1024 + (16 * h + col) * 32 + 4 * st + kq];
         }
-        const long long g = std::min<long long>(grid, (sp.numTiles + 7) / 8 * 8);
-        sp.tilesPerGroup = (sp.numTiles + g - 1) / g;
-        hipLaunchKernelGGL(kFused5Tile, dim3((unsigned)g), dim3(kBlock), 0, hstream(), (Amp<float> *)q->amps, sp, M);
-        checkLaunch("fused5Apply");
-        return;
     }
+    uint32_t *lanes = reinterpret_cast<uint32_t *>(frag + kFragVecs * 64 * 4);
+    for (int tid = 0; tid < kBlock; ++tid) {
+        const int wave = tid >> 6, lane = tid & 63, col = lane & 15, kq = lane >> 4;
+        const unsigned g = 16 * wave + col;
+        unsigned eG = 0;
+        for (int b = 0; b < 6; ++b) eG |= ((g >> b) & 1u) << sp.gIdx[b];
+        const unsigned inLane = tileSlot(eG | tileCombo(kq, sp), sp);
+        const unsigned outLane = tileSlot(eG | tileCombo(4 * kq, sp), sp);
+        const unsigned eLane = 2 * tid;
+        uint64_t laneAddr = eLane & 63u;
+        for (int i = 0; i < 3; ++i) laneAddr |= (uint64_t)((eLane >> (6 + i)) & 1u) << sp.pos[6 + i];
+        lanes[4 * tid + 0] = inLane | (outLane << 16);
+        lanes[4 * tid + 1] = tileSlot(eLane, sp);
+        lanes[4 * tid + 2] = (uint32_t)laneAddr;
+        lanes[4 * tid + 3] = (uint32_t)(laneAddr >> 32);
+    }
+}
+
+void launchFused5Tile(QuregDev *q, const Fused5Spec &spec, const void *form) {
+    Fused5TileSpec sp = tileSpecOf(spec.targetsSorted);
+    sp.numTiles = q->numAmps / kTileAmps;
+    sp.tilesPerGroup = kTilesPerGroup;
+    // a multiple of 8 workgroups (xcdBlock's XCD order); the extra ones exit
+    const long long g = ((sp.numTiles + kTilesPerGroup - 1) / kTilesPerGroup + 7) / 8 * 8;
+    hipLaunchKernelGGL(kFused5Tile, dim3((unsigned)g), dim3(kBlock), 0, hstream(), (Amp<float> *)q->amps, sp,
+                       static_cast<const f4v *>(form));
+    checkLaunch("fused5Apply");
+}
+
+void launchFused5Direct(QuregDev *q, const Fused5Spec &spec, const float *M) {
     const long long numChunks = (q->numAmps >> 5) / 16;
     const long long wavesNeeded = (numChunks + kChunksPerWave - 1) / kChunksPerWave;
     const long long grid = (wavesNeeded + (kBlock / 64) - 1) / (kBlock / 64);
@@ -409,34 +431,51 @@ void launchFused5(QuregDev *q, const Fused5Spec &spec, const float *M) {
 
 }  // namespace
 
-// Build the device-resident form of a 5-qubit block once, for blocks applied
-// many times (circuits).
+// Build the device-resident forms of a 5-qubit block once, for blocks applied
+// many times (circuits): the row-major matrices (registers of 9-10 qubits) and
+// the tile form (11 qubits and more).
 void *fused5Prepare(const int *targs, const Cplx *u) {
     auto *p = new Fused5Prepared;
     std::vector<float> hm(kFused5MatrixFloats);
     fused5Build(targs, u, p->spec, hm.data());
+    std::vector<char> form(kFused5TileFormBytes);
+    tileFormBuild(hm.data(), tileSpecOf(p->spec.targetsSorted), form.data());
     p->M = static_cast<float *>(devAlloc(hm.size() * sizeof(float)));
     hostToDevice(p->M, hm.data(), hm.size() * sizeof(float));
+    p->form = devAlloc(form.size());
+    hostToDevice(p->form, form.data(), form.size());
     return p;
 }
 
 // One application of a block (multiQubitUnitary on an FP32 register): the
-// matrix goes through the staging ring, with no allocation or synchronisation.
+// form the register size needs goes through the staging ring, with no
+// allocation or synchronisation.
 void fused5ApplyOnce(QuregDev *q, const int *targs, const Cplx *u) {
     constexpr size_t bytes = kFused5MatrixFloats * sizeof(float);
-    static_assert(bytes <= kStageSlotBytes, "a fused block fits a staging slot");
+    static_assert(bytes <= kStageSlotBytes && kFused5TileFormBytes <= kStageSlotBytes,
+                  "a fused block fits a staging slot");
     Fused5Spec spec;
     hipStream_t st = hstream();
+    if (q->numQubits >= kTileLog) {
+        float hm[kFused5MatrixFloats];
+        fused5Build(targs, u, spec, hm);
+        const Staged s = stageBegin(kFused5TileFormBytes);
+        tileFormBuild(hm, tileSpecOf(spec.targetsSorted), s.host);
+        stageCommit(s, kFused5TileFormBytes, st);
+        launchFused5Tile(q, spec, s.dev);
+        return;
+    }
     const Staged s = stageBegin(bytes);
     fused5Build(targs, u, spec, static_cast<float *>(s.host));
     stageCommit(s, bytes, st);
-    launchFused5(q, spec, static_cast<const float *>(s.dev));
+    launchFused5Direct(q, spec, static_cast<const float *>(s.dev));
 }
 
 void fused5Free(void *prepared) {
     auto *p = static_cast<Fused5Prepared *>(prepared);
     if (!p) return;
     devFree(p->M);
+    devFree(p->form);
     delete p;
 }
 
@@ -444,7 +483,8 @@ void fused5Free(void *prepared) {
 // chunk is 512 amplitudes).
 void fused5Apply(QuregDev *q, const void *prepared) {
     const auto *p = static_cast<const Fused5Prepared *>(prepared);
-    launchFused5(q, p->spec, p->M);
+    if (q->numQubits >= kTileLog) launchFused5Tile(q, p->spec, p->form);
+    else launchFused5Direct(q, p->spec, p->M);
 }
 
 }  // namespace qmx
