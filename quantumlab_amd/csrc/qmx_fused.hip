@@ -33,6 +33,7 @@
 // form: 256), which keeps the matrix cores well inside the time MI355X's HBM
 // needs for the sweep; every fused block replaces ~10 per-gate sweeps.
 #include <algorithm>
+#include <cstdio>
 #include <vector>
 
 #include "qmx_device.h"
@@ -178,7 +179,8 @@ struct Fused5TileSpec {
     long long numTiles, tilesPerGroup;
 };
 
-__host__ __device__ __forceinline__ unsigned tileSlot(unsigned e, const Fused5TileSpec &sp) {
+template <class Spec>
+__host__ __device__ __forceinline__ unsigned tileSlot(unsigned e, const Spec &sp) {
     unsigned s = 0;
 #pragma unroll
     for (int i = 0; i < kTileLog; ++i)
@@ -186,7 +188,8 @@ __host__ __device__ __forceinline__ unsigned tileSlot(unsigned e, const Fused5Ti
     return s;
 }
 
-__host__ __device__ __forceinline__ unsigned tileCombo(unsigned c, const Fused5TileSpec &sp) {
+template <class Spec>
+__host__ __device__ __forceinline__ unsigned tileCombo(unsigned c, const Spec &sp) {
     unsigned e = 0;
 #pragma unroll
     for (int j = 0; j < 5; ++j) e |= ((c >> j) & 1u) << sp.tIdx[j];
@@ -194,7 +197,8 @@ __host__ __device__ __forceinline__ unsigned tileCombo(unsigned c, const Fused5T
 }
 
 // state index of tile t's element 0: t with zeros inserted at the P positions
-__device__ __forceinline__ uint64_t tileBase(uint64_t t, const Fused5TileSpec &sp) {
+template <class Spec>
+__device__ __forceinline__ uint64_t tileBase(uint64_t t, const Spec &sp) {
 #pragma unroll
     for (int i = 0; i < kTileLog; ++i) t = insertZero(t, sp.pos[i]);
     return t;
@@ -296,6 +300,110 @@ __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict_
     }
 }
 
+// ---- FP64 tile form ------------------------------------------------------------
+// The same tiles on FP64 registers (2048 amplitudes = 32 KiB, two per
+// workgroup in LDS, so two workgroups per CU), multiplied with
+// v_mfma_f64_16x16x4f64 in the same three-product form. Its result element v
+// of row tile h is combination 16 h + 4 v + kq, the input combination of
+// step 4 h + v: a lane writes each result to the LDS slot it read an input
+// from. A thread moves elements e = tid + 256 j (j < 8), one amplitude per 16
+// B access: bits 0-7 of e from tid (qubits 0-5, then P[6], P[7]), bits 8-10
+// from j. The block's complex matrix (ascending-target order, 16 KiB) is the
+// only per-block input; each lane reads its 16 fragments from it once.
+struct Fused5TileSpec64 {
+    int pos[kTileLog];
+    int tIdx[5];
+    int gIdx[6];
+    unsigned slotBit[kTileLog];
+    unsigned sIn[8], jSlot[8];
+    uint64_t jAddr[8];
+    long long numTiles, tilesPerGroup;
+};
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(kBlock, 2) kFused5Tile64(Amp<double> *__restrict__ a, Fused5TileSpec64 sp,
+                                                        const d2v *__restrict__ M) {
+    __shared__ __attribute__((aligned(16))) Amp<double> buf[2][kTileAmps];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15, kq = lane >> 4;
+    const long long first = xcdBlock() * sp.tilesPerGroup;
+    const long long last = min(first + sp.tilesPerGroup, sp.numTiles);
+    if (first >= last) return;  // the same for every thread of the workgroup
+
+    // fragments of row tile h, step st: M[16h + col][4st + kq] (real and
+    // imaginary; their sum, the third product's operand, is formed per use)
+    double fr[2][8], fi[2][8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            const d2v z = M[(16 * h + col) * 32 + 4 * st + kq];
+            fr[h][st] = z.x;
+            fi[h][st] = z.y;
+        }
+    const unsigned g = 16 * wave + col;
+    unsigned eG = 0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) eG |= ((g >> b) & 1u) << sp.gIdx[b];
+    const unsigned inLane = tileSlot(eG | tileCombo(kq, sp), sp);
+    uint64_t laneAddr = tid & 63u;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) laneAddr |= (uint64_t)((tid >> (6 + i)) & 1u) << sp.pos[6 + i];
+    const unsigned laneSlot = tileSlot((unsigned)tid, sp);
+
+    auto load = [&](d2v (&R)[8], long long t) {
+        const uint64_t base = tileBase((uint64_t)t, sp);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            R[j] = __builtin_nontemporal_load(reinterpret_cast<const d2v *>(a + ((base | laneAddr) | sp.jAddr[j])));
+    };
+    auto step = [&](d2v (&R)[8], long long t, Amp<double> *T) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) T[laneSlot ^ sp.jSlot[j]] = Amp<double>{R[j].x, R[j].y};
+        __syncthreads();
+        if (t + 1 < last) load(R, t + 1);
+        d4v acc[2][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int p = 0; p < 3; ++p) acc[h][p] = d4v{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            const Amp<double> x = T[inLane ^ sp.sIn[st]];
+            const double xs = x.x + x.y;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                acc[h][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr[h][st], x.x, acc[h][0], 0, 0, 0);
+                acc[h][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fi[h][st], x.y, acc[h][1], 0, 0, 0);
+                acc[h][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(fr[h][st] + fi[h][st], xs, acc[h][2], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                T[inLane ^ sp.sIn[4 * h + v]] = Amp<double>{acc[h][0][v] - acc[h][1][v],
+                                                            acc[h][2][v] - acc[h][0][v] - acc[h][1][v]};
+        __syncthreads();
+        const uint64_t cur = tileBase((uint64_t)t, sp);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const Amp<double> o = T[laneSlot ^ sp.jSlot[j]];
+            __builtin_nontemporal_store(d2v{o.x, o.y}, reinterpret_cast<d2v *>(a + ((cur | laneAddr) | sp.jAddr[j])));
+        }
+    };
+    // one tile's loads in flight (the next tile is requested as soon as the
+    // current one is in LDS); LDS buffer = tile parity
+    d2v R[8];
+    load(R, first);
+    for (long long t = first; t < last; t += 2) {
+        step(R, t, buf[0]);
+        if (t + 1 < last) step(R, t + 1, buf[1]);
+    }
+}
+
 struct Fused5Prepared {
     Fused5Spec spec;
     float *M;     // device copy of the block's three real 32x32 matrices (Ur, Ui, Ur + Ui)
@@ -307,7 +415,7 @@ constexpr int kFused5MatrixFloats = 3 * 32 * 32;
 // The kernel's form of a 5-qubit block (targs[0] = least significant bit of
 // u's index, u row-major 32x32): the spec, and the three real 32x32 matrices
 // Ur, Ui and Ur + Ui (row-major, one after the other) written to hm.
-void fused5Build(const int *targs, const Cplx *u, Fused5Spec &spec, float *hm) {
+void fused5Order(const int *targs, Fused5Spec &spec, int toOld[32]) {
     // index the block by ascending target: bit j of a combination is the j-th
     // lowest target, so a wave's 4 combinations (MFMA K lanes) run over the
     // lowest targets and its loads cover whole cache lines
@@ -315,7 +423,6 @@ void fused5Build(const int *targs, const Cplx *u, Fused5Spec &spec, float *hm) {
     std::sort(order, order + 5, [&](int x, int y) { return targs[x] < targs[y]; });
     spec.targetsSorted.n = 5;
     for (int j = 0; j < 5; ++j) spec.targetsSorted.pos[j] = targs[order[j]];
-    int toOld[32];
     for (int c = 0; c < 32; ++c) {
         uint64_t o = 0;
         int old = 0;
@@ -327,6 +434,11 @@ void fused5Build(const int *targs, const Cplx *u, Fused5Spec &spec, float *hm) {
         spec.off[c] = o;
         toOld[c] = old;
     }
+}
+
+void fused5Build(const int *targs, const Cplx *u, Fused5Spec &spec, float *hm) {
+    int toOld[32];
+    fused5Order(targs, spec, toOld);
     for (int r = 0; r < 32; ++r)
         for (int c = 0; c < 32; ++c) {
             const Cplx z = u[toOld[r] * 32 + toOld[c]];
@@ -420,6 +532,54 @@ void launchFused5Tile(QuregDev *q, const Fused5Spec &spec, const void *form) {
     checkLaunch("fused5Apply");
 }
 
+// The FP64 tile form's spec: the positions of tileSpecOf and a swizzle for
+// 16 B LDS accesses (8 lanes per 128 B bank cycle): the slots of elements 1, 2
+// and 4 (eight consecutive threads' transfer elements) and those of group bits
+// 0-2 (eight lanes of one operand read) each span all 8 slots modulo 8.
+// slotBit[i] = 2^i plus bits below min(i, 3) keeps the map invertible.
+Fused5TileSpec64 tileSpecOf64(const BitList &targets) {
+    const Fused5TileSpec base = tileSpecOf(targets);
+    Fused5TileSpec64 sp{};
+    for (int i = 0; i < kTileLog; ++i) {
+        sp.pos[i] = base.pos[i];
+        sp.slotBit[i] = 1u << i;
+    }
+    for (int j = 0; j < 5; ++j) sp.tIdx[j] = base.tIdx[j];
+    for (int b = 0; b < 6; ++b) sp.gIdx[b] = base.gIdx[b];
+    auto rank3 = [](unsigned x, unsigned y, unsigned z) {
+        return x && y && z && (x ^ y) && (x ^ z) && (y ^ z) && (x ^ y ^ z);
+    };
+    bool found = false;
+    for (unsigned x0 = 0; x0 < 8 && !found; ++x0)
+        for (unsigned x1 = 0; x1 < 8 && !found; ++x1)
+            for (unsigned x2 = 0; x2 < 8 && !found; ++x2) {
+                const unsigned xs[3] = {x0, x1, x2};
+                bool ok = true;
+                unsigned img[3];
+                for (int b = 0; b < 3; ++b) {
+                    const int i = sp.gIdx[b];
+                    const unsigned allowed = (1u << std::min(i, 3)) - 1u;  // bits below min(i, 3)
+                    if (xs[b] & ~allowed) ok = false;
+                    img[b] = ((1u << i) & 7u) ^ xs[b];
+                }
+                if (ok && rank3(img[0], img[1], img[2])) {
+                    for (int b = 0; b < 3; ++b) sp.slotBit[sp.gIdx[b]] ^= xs[b];
+                    found = true;
+                }
+            }
+    if (!found) {
+        std::fprintf(stderr, "libquest_mi355x: no FP64 tile swizzle for these targets\n");
+        std::abort();
+    }
+    for (int st = 0; st < 8; ++st) sp.sIn[st] = tileSlot(tileCombo(4 * st, sp), sp);
+    for (int j = 0; j < 8; ++j) {
+        sp.jAddr[j] = ((uint64_t)(j & 1) << sp.pos[8]) | ((uint64_t)((j >> 1) & 1) << sp.pos[9]) |
+                      ((uint64_t)(j >> 2) << sp.pos[10]);
+        sp.jSlot[j] = tileSlot(256u * j, sp);
+    }
+    return sp;
+}
+
 void launchFused5Direct(QuregDev *q, const Fused5Spec &spec, const float *M) {
     const long long numChunks = (q->numAmps >> 5) / 16;
     const long long wavesNeeded = (numChunks + kChunksPerWave - 1) / kChunksPerWave;
@@ -485,6 +645,33 @@ void fused5Apply(QuregDev *q, const void *prepared) {
     const auto *p = static_cast<const Fused5Prepared *>(prepared);
     if (q->numQubits >= kTileLog) launchFused5Tile(q, p->spec, p->form);
     else launchFused5Direct(q, p->spec, p->M);
+}
+
+// One application of an uncontrolled 5-qubit matrix to an FP64 register of
+// >= 11 qubits: the block's complex matrix in ascending-target order goes
+// through the staging ring (16 KiB), with no allocation or synchronisation.
+void fused5ApplyOnce64(QuregDev *q, const int *targs, const Cplx *u) {
+    constexpr size_t bytes = 32 * 32 * sizeof(Amp<double>);
+    static_assert(bytes <= kStageSlotBytes, "an FP64 block fits a staging slot");
+    Fused5Spec spec;
+    int toOld[32];
+    fused5Order(targs, spec, toOld);
+    hipStream_t st = hstream();
+    const Staged sg = stageBegin(bytes);
+    Amp<double> *hm = static_cast<Amp<double> *>(sg.host);
+    for (int r = 0; r < 32; ++r)
+        for (int c = 0; c < 32; ++c) {
+            const Cplx z = u[toOld[r] * 32 + toOld[c]];
+            hm[r * 32 + c] = Amp<double>{z.re, z.im};
+        }
+    stageCommit(sg, bytes, st);
+    Fused5TileSpec64 sp = tileSpecOf64(spec.targetsSorted);
+    sp.numTiles = q->numAmps / kTileAmps;
+    sp.tilesPerGroup = kTilesPerGroup;
+    const long long g = ((sp.numTiles + kTilesPerGroup - 1) / kTilesPerGroup + 7) / 8 * 8;
+    hipLaunchKernelGGL(kFused5Tile64, dim3((unsigned)g), dim3(kBlock), 0, st, (Amp<double> *)q->amps, sp,
+                       static_cast<const d2v *>(sg.dev));
+    checkLaunch("fused5ApplyOnce64");
 }
 
 }  // namespace qmx

@@ -517,6 +517,11 @@ void applyMatrixN(QuregDev *q, uint64_t ctrlMask, const int *targs, int k, const
         fused5ApplyOnce(q, targs, u);
         return;
     }
+    if (q->prec == FP64 && k == 5 && ctrlMask == 0 && n >= 11) {
+        // and on FP64 registers large enough for whole tiles, the FP64 MFMA tiles
+        fused5ApplyOnce64(q, targs, u);
+        return;
+    }
     if (ctrlMask == 0 && denseBitsSupported(k) && n >= k) {
         // uncontrolled 2..5-target matrices: one group per thread in registers
         applyDense(q, targs, k, u);

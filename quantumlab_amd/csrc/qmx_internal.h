@@ -124,6 +124,8 @@ void fused5Free(void *prepared);
 void fused5Apply(QuregDev *q, const void *prepared);
 // one application without a prepared form (staged per call, no allocation)
 void fused5ApplyOnce(QuregDev *q, const int *targs, const Cplx *u);
+// uncontrolled 5-qubit matrix on an FP64 register of >= 11 qubits (MFMA tiles)
+void fused5ApplyOnce64(QuregDev *q, const int *targs, const Cplx *u);
 // exp(-i angle/2 * P), P = X^{xMask} Z^{zMask} with Y = iXZ folded in (single sweep)
 void applyPauliRotation(QuregDev *q, uint64_t ctrlMask, uint64_t xMask, uint64_t zMask,
                         int numY, double angle);

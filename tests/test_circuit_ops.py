@@ -213,10 +213,11 @@ def test_circuit_apply_rejects_other_widths(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,k", [(14, 5), (16, 5), (16, 3)])
+@pytest.mark.parametrize("n,k", [(10, 5), (14, 5), (16, 5), (16, 3)])
 def test_fused_brick_fp64_matches_oracle(gpu, cpuref, n, k):
     """Brick circuit with the 3-control and 5-target extras, fused into dense
-    FP64 blocks (kDense for 2-5 qubits), against the oracle's per-gate run."""
+    FP64 blocks (5-qubit blocks on the FP64 MFMA tiles from 11 qubits, kDense
+    for 2-4 qubits and below 11), against the oracle's per-gate run."""
     gates = W.brick_circuit(n, 10, seed=40 + n, extra_every=5)
     qc = cpuref.qureg(n)
     W.apply_gates(cpuref, qc, gates)

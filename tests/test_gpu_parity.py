@@ -566,6 +566,32 @@ def test_prob_of_all_outcomes_matches_oracle(gpu, cpuref, density, n):
 
 @pytest.mark.parametrize("n,targets", [(11, [0, 1, 2, 3, 4]), (11, [6, 7, 8, 9, 10]), (12, [0, 3, 7, 10, 11]),
                                        (13, [4, 0, 12, 2, 1]), (14, [5, 6, 7, 8, 9]), (14, [13, 0, 9, 1, 12]),
+                                       (15, [0, 1, 2, 5, 9]), (15, [3, 1, 14, 6, 2]), (12, [1, 2, 3, 4, 5]),
+                                       (10, [9, 0, 4, 2, 7])])
+def test_fp64_five_target_blocks_every_placement(gpu, cpuref, n, targets):
+    """Uncontrolled 5-target multiQubitUnitary on FP64 registers (the FP64
+    MFMA tiles from 11 qubits, kDense below) against the oracle's
+    multiQubitUnitary on the same state, for low, high, mid, scattered and
+    unsorted targets (low group bits in every order of the LDS swizzle)."""
+    rng = np.random.default_rng(100 + n + sum(targets))
+    psi = random_state(n, 5 + n)
+    regs = []
+    for lib in (gpu, cpuref):
+        q = lib.qureg(n)
+        lib.set_state(q, psi)
+        regs.append(q)
+    for _ in range(3):
+        u = W.haar_unitary(rng, 32)
+        for lib, q in zip((gpu, cpuref), regs):
+            lib.multiQubitUnitary(q, targets, u)
+    got, want = gpu.state(regs[0]), cpuref.state(regs[1])
+    assert np.max(np.abs(got - want)) < 1e-12
+    gpu.destroy(regs[0])
+    cpuref.destroy(regs[1])
+
+
+@pytest.mark.parametrize("n,targets", [(11, [0, 1, 2, 3, 4]), (11, [6, 7, 8, 9, 10]), (12, [0, 3, 7, 10, 11]),
+                                       (13, [4, 0, 12, 2, 1]), (14, [5, 6, 7, 8, 9]), (14, [13, 0, 9, 1, 12]),
                                        (10, [9, 0, 4, 2, 7]), (9, [0, 1, 2, 3, 4])])
 def test_fp32_five_target_blocks_every_placement(gpu, n, targets):
     """Uncontrolled 5-target multiQubitUnitary on FP32 registers (the fused
