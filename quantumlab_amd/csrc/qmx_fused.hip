@@ -310,6 +310,11 @@ __global__ void __launch_bounds__(kBlock, 3) kFused5Tile(Amp<float> *__restrict_
 // B access: bits 0-7 of e from tid (qubits 0-5, then P[6], P[7]), bits 8-10
 // from j. The block's complex matrix (ascending-target order, 16 KiB) is the
 // only per-block input; each lane reads its 16 fragments from it once.
+// tiles per workgroup (32 KiB each): at 32 qubits 8, 16 and 32 measured
+// 22.4-23.1 ms per block, 2 and 4 26.0-26.1 ms (repeat runs of one setting
+// vary by up to 4 ms with the register's placement in memory)
+constexpr int kTilesPerGroup64 = 8;
+
 struct Fused5TileSpec64 {
     int pos[kTileLog];
     int tIdx[5];
@@ -667,8 +672,8 @@ void fused5ApplyOnce64(QuregDev *q, const int *targs, const Cplx *u) {
     stageCommit(sg, bytes, st);
     Fused5TileSpec64 sp = tileSpecOf64(spec.targetsSorted);
     sp.numTiles = q->numAmps / kTileAmps;
-    sp.tilesPerGroup = kTilesPerGroup;
-    const long long g = ((sp.numTiles + kTilesPerGroup - 1) / kTilesPerGroup + 7) / 8 * 8;
+    sp.tilesPerGroup = kTilesPerGroup64;
+    const long long g = ((sp.numTiles + kTilesPerGroup64 - 1) / kTilesPerGroup64 + 7) / 8 * 8;
     hipLaunchKernelGGL(kFused5Tile64, dim3((unsigned)g), dim3(kBlock), 0, st, (Amp<double> *)q->amps, sp,
                        static_cast<const d2v *>(sg.dev));
     checkLaunch("fused5ApplyOnce64");
