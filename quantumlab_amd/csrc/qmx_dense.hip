@@ -334,7 +334,7 @@ constexpr int kMfmaBatches = 16 / (kBlock / 64);  // 16-group batches per wave a
 template <typename T>
 __global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<T> *__restrict__ a, unsigned numTiles, DensePairSpec sp,
                                                          const Amp<T> *__restrict__ MA,
-                                                         const Amp<T> *__restrict__ MB) {
+                                                         const Amp<T> *__restrict__ MB, unsigned perGroup) {
     constexpr int PS = sizeof(T) == 8 ? 3 : 4;
     extern __shared__ unsigned char smemRaw[];
     Amp<T> *lds = reinterpret_cast<Amp<T> *>(smemRaw);
@@ -356,9 +356,13 @@ __global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<T> *__restrict__ a,
             for (int s = 0; s < 4; ++s) x[i][s] = src[bg | LA.g[s]];
         }
     };
-    unsigned w = blockIdx.x;
-    if (w < numTiles) loadInputs(w);
-    for (; w < numTiles; w += gridDim.x) {
+    // perGroup = 0: a persistent grid striding over the tiles; otherwise
+    // perGroup consecutive tiles per workgroup (XCD order)
+    unsigned w = perGroup ? (unsigned)xcdBlock() * perGroup : blockIdx.x;
+    const unsigned wEnd = perGroup ? min(numTiles, w + perGroup) : numTiles;
+    const unsigned stride = perGroup ? 1u : gridDim.x;
+    if (w < wEnd) loadInputs(w);
+    for (; w < wEnd; w += stride) {
         // first superoperator: registers -> LDS
 #pragma unroll
         for (int i = 0; i < kMfmaBatches; ++i) {
@@ -373,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) kDensePairMfma(Amp<T> *__restrict__ a,
                 lds[j + (j >> PS)] = Amp<T>{re[v], im[v]};
             }
         }
-        if (w + gridDim.x < numTiles) loadInputs(w + gridDim.x);
+        if (w + stride < wEnd) loadInputs(w + stride);
         __syncthreads();
         // second superoperator: LDS -> HBM
         Amp<T> *dst = a + pairTileBase((uint64_t)w, sp.tile);
@@ -531,11 +535,20 @@ void launchDensePair(QuregDev *q, const DensePairSpec &sp, const Cplx *MA, const
     std::copy(MB, MB + (1 << (2 * BB)), both + 256);
     const Amp<T> *m = stageMatrix<T>(both, 512, st);
     const unsigned grid = (unsigned)(q->numAmps >> kPairTileBits);
-    // persistent: FP32 3 workgroups per CU (its occupancy), FP64 2 (its LDS)
-    if constexpr (mfma)
-        hipLaunchKernelGGL(kDensePairMfma<T>, dim3(std::min(grid, (sizeof(T) == 8 ? 2u : 3u) * (unsigned)numCUs())),
-                           dim3(kBlock), lds, st, (Amp<T> *)q->amps, grid, sp, m, m + 256);
-    else
+    // FP32: a persistent grid of 3 workgroups per CU (its occupancy); FP64:
+    // 2 consecutive tiles per workgroup (at 14 qubits 0.36 s for the noisy
+    // circuit against 0.39 s persistent, 2 workgroups per CU by its LDS; 4,
+    // 8 and 16 tiles 0.37-0.38 s; FP32 gained nothing from it)
+    if constexpr (mfma) {
+        if constexpr (sizeof(T) == 8) {
+            constexpr unsigned per = 2;
+            hipLaunchKernelGGL(kDensePairMfma<T>, dim3(((grid + per - 1) / per + 7u) / 8u * 8u), dim3(kBlock), lds,
+                               st, (Amp<T> *)q->amps, grid, sp, m, m + 256, per);
+        } else {
+            hipLaunchKernelGGL(kDensePairMfma<T>, dim3(std::min(grid, 3u * (unsigned)numCUs())), dim3(kBlock), lds,
+                               st, (Amp<T> *)q->amps, grid, sp, m, m + 256, 0u);
+        }
+    } else
         hipLaunchKernelGGL((kDensePair<T, BA, BB>), dim3(grid), dim3(kBlock), lds, st, (Amp<T> *)q->amps, sp, m,
                            m + 256);
 }
