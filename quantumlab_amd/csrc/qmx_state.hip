@@ -5,6 +5,7 @@
 // accesses, XCD-ordered workgroups.
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "qmx_device.h"
 
@@ -88,17 +89,24 @@ __global__ void kDeinterleave(const Amp<T> *a, H *re, H *im, long long n) {
         }
 }
 
-// out = fac1*q1 + fac2*q2 + facOut*out (QuEST_cpu.c statevec_setWeightedQureg)
+// out = fac1*q1 + fac2*q2 + facOut*out with QuEST's grouping and every
+// operation rounded separately, (facOut out) + (fac1 q1) + (fac2 q2), each
+// product's two terms formed first (QuEST_cpu.c:3988-4027, built without
+// contraction): the same inputs give the same bits. In qreal, in float only
+// when both the registers and qreal are float.
 template <typename T>
 __global__ void kWeighted(Cplx f1, const Amp<T> *a, Cplx f2, const Amp<T> *b, Cplx fo, Amp<T> *o,
                           long long n) {
+#pragma clang fp contract(off)
+    using A = typename std::conditional<sizeof(T) == 4 && sizeof(qreal) == 4, float, double>::type;
+    const A f1r = (A)f1.re, f1i = (A)f1.im, f2r = (A)f2.re, f2i = (A)f2.im, for_ = (A)fo.re, foi = (A)fo.im;
     long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) {
-            double ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y, orr = o[i].x, oi = o[i].y;
-            double re = fo.re * orr - fo.im * oi + f1.re * ar - f1.im * ai + f2.re * br - f2.im * bi;
-            double im = fo.re * oi + fo.im * orr + f1.re * ai + f1.im * ar + f2.re * bi + f2.im * br;
+            const A ar = a[i].x, ai = a[i].y, br = b[i].x, bi = b[i].y, orr = o[i].x, oi = o[i].y;
+            const A re = (for_ * orr - foi * oi) + (f1r * ar - f1i * ai) + (f2r * br - f2i * bi);
+            const A im = (for_ * oi + foi * orr) + (f1r * ai + f1i * ar) + (f2r * bi + f2i * br);
             o[i] = Amp<T>{(T)re, (T)im};
         }
 }
@@ -106,13 +114,17 @@ __global__ void kWeighted(Cplx f1, const Amp<T> *a, Cplx f2, const Amp<T> *b, Cp
 // combine = (1-p) combine + p other (QuEST_cpu.c:908-936)
 template <typename T>
 __global__ void kMixDens(Amp<T> *c, const Amp<T> *o, double p, long long n) {
+    // in qreal, each operation rounded separately, as QuEST's loop
+#pragma clang fp contract(off)
+    using A = typename std::conditional<sizeof(T) == 4 && sizeof(qreal) == 4, float, double>::type;
+    const A pa = (A)p, q = (A)1 - pa;
     long long i = flatBlockLinear() * kBlock * 4 + threadIdx.x;
 #pragma unroll
     for (int j = 0; j < 4; ++j, i += kBlock)
         if (i < n) {
-            double cr = c[i].x * (1 - p), ci = c[i].y * (1 - p);
-            cr += p * o[i].x;
-            ci += p * o[i].y;
+            A cr = (A)c[i].x * q, ci = (A)c[i].y * q;
+            cr += pa * (A)o[i].x;
+            ci += pa * (A)o[i].y;
             c[i] = Amp<T>{(T)cr, (T)ci};
         }
 }

@@ -453,3 +453,61 @@ def test_full_api_validation_messages(gpu):
     gpu.lib.destroyDiagonalOp(op, gpu.env)
     gpu.destroy(q)
     gpu.destroy(rho)
+
+
+# ---- combinations of registers, bit for bit against QuEST's own build -------------------
+
+@pytest.mark.gpu
+def test_weighted_and_mixed_registers_bit_identical_to_reference(gpu, reflib):
+    """setWeightedQureg and mixDensityMatrix round every operation in QuEST's
+    order (QuEST_cpu.c:3988-4027, 908-936), so the same inputs give the same
+    bits as QuEST's CPU backend built without contraction. Values as in QuEST's
+    suite (test_state_initialisations.cpp:540-610: entries and factors drawn
+    from [-5, 5], the output aliasing each input in turn, then all three), where
+    three successive weightings grow entries past 1e3 and its 1e-12 tolerance
+    is below one ulp."""
+    from quantumlab_amd.quest import cplx
+    n = 5
+    rng = np.random.default_rng(592)
+    vecs = [rng.uniform(-5, 5, 1 << n) + 1j * rng.uniform(-5, 5, 1 << n) for _ in range(3)]
+    regs = {}
+    for name, lib in (("gpu", gpu), ("ref", reflib)):
+        qs = []
+        for v in vecs:
+            q = lib.qureg(n)
+            lib.set_state(q, v)
+            qs.append(q)
+        regs[name] = qs
+    facs = [cplx(complex(rng.uniform(-5, 5), rng.uniform(-5, 5))) for _ in range(3)]
+    fa, fb, fc = facs
+    patterns = [(fa, 0, fb, 1, fc, 2), (fb, 1, fc, 2, fa, 2), (fc, 2, fb, 1, fa, 2), (fa, 2, fb, 2, fc, 2)]
+    for f1, i1, f2, i2, fo, io in patterns:
+        for name, lib in (("gpu", gpu), ("ref", reflib)):
+            qs = regs[name]
+            lib.call("setWeightedQureg", f1, qs[i1], f2, qs[i2], fo, qs[io])
+        for k in range(3):
+            assert np.array_equal(gpu.state(regs["gpu"][k]), reflib.state(regs["ref"][k])), (k, io)
+    assert np.max(np.abs(reflib.state(regs["ref"][2]))) > 1e3
+    for name, lib in (("gpu", gpu), ("ref", reflib)):
+        for q in regs[name]:
+            lib.destroy(q)
+    # mixDensityMatrix on two density registers holding the same arbitrary bits
+    m = 4
+    flats = [rng.uniform(-1, 1, 4 ** m) + 1j * rng.uniform(-1, 1, 4 ** m) for _ in range(2)]
+    rhos = {}
+    for name, lib in (("gpu", gpu), ("ref", reflib)):
+        pair = []
+        for f in flats:
+            q = lib.qureg(m, density=True)
+            re, im = lib.reals(f.real), lib.reals(f.imag)
+            lib.call("setDensityAmps", q, 0, 0, re.ctypes.data_as(lib.real_ptr), im.ctypes.data_as(lib.real_ptr),
+                     4 ** m)
+            pair.append(q)
+        a, b = pair
+        lib.call("mixDensityMatrix", a, 0.37, b)
+        lib.call("mixDensityMatrix", a, 0.11, b)
+        rhos[name] = (a, b)
+    assert np.array_equal(gpu.state(rhos["gpu"][0]), reflib.state(rhos["ref"][0]))
+    for name, lib in (("gpu", gpu), ("ref", reflib)):
+        for q in rhos[name]:
+            lib.destroy(q)
