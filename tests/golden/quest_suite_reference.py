@@ -5,7 +5,7 @@ The suite (Code/QuEST340/tests) draws its random states and matrices from
 rand() without seeding, so a given invocation is deterministic; a few of its
 sections fail on the reference build itself (a random 5-qubit matrix that
 misses the 1e-13 unitarity check, a 1e-13 comparison, distribution-limit
-checks whose rank count overflows int). tests/test_quest_suite.py runs the
+checks whose rank count overflows int). tests/test_yy_quest_suite.py runs the
 same tags the same way against this library and accepts these.
 
 Flavours: f64 (QuEST_PREC=2) and f32 (QuEST_PREC=1).

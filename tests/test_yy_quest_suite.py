@@ -43,7 +43,10 @@ the draws. Every density operation is compared with QuEST's CPU backend at
 Every other failure, an exception where none is due, a missing error, a wrong
 message or a crash, fails the test.
 
-The binaries travel with the repository."""
+The binaries travel with the repository. This file runs after the other GPU
+tests (only the allocation-free entry point's test follows it): its sections
+draw random inputs, so a new kind of random miss stops a `pytest -x` run here
+rather than before the parity tests."""
 import json
 import os
 import re
