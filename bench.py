@@ -790,7 +790,7 @@ def main():
             args.fp32_qubits = 0
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the state-vector "
-              f"path here has no multi-GPU sharding; see DESIGN.md for how it would shard)", file=sys.stderr)
+              f"path here has no multi-GPU sharding: one register lives on one GPU)", file=sys.stderr)
         sys.exit(2)
 
     if not args.no_cpu:
