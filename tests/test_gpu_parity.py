@@ -590,6 +590,38 @@ def test_fp64_five_target_blocks_every_placement(gpu, cpuref, n, targets):
     cpuref.destroy(regs[1])
 
 
+@pytest.mark.parametrize("n,targets", [(6, [0, 1, 2, 3, 4]), (6, [5, 1, 3, 0, 2]), (7, [6, 0, 3, 5, 1])])
+def test_five_target_blocks_on_density_registers(gpu, cpuref, n, targets):
+    """multiQubitUnitary with 5 targets on density registers of 6-7 qubits
+    (12-14-qubit vectors: the tile kernels on the rows, then on the columns
+    with the conjugate matrix, QuEST.c:302-317): FP64 against the oracle at
+    1e-12; an FP32-storage register against the FP64 one within FP32
+    resolution."""
+    rng = np.random.default_rng(300 + n + sum(targets))
+    regs = []
+    for lib in (gpu, cpuref):
+        rho, pure = lib.qureg(n, density=True), lib.qureg(n)
+        channel_circuit(lib, rho, pure, n, 2, seed=n)
+        lib.destroy(pure)
+        regs.append(rho)
+    us = [W.haar_unitary(rng, 32) for _ in range(2)]
+    for lib, rho in zip((gpu, cpuref), regs):
+        for u in us:
+            lib.multiQubitUnitary(rho, targets, u)
+    got, want = gpu.state(regs[0]), cpuref.state(regs[1])
+    assert np.max(np.abs(got - want)) < 1e-12
+    r32 = gpu.qureg(n, density=True, precision=QMX_PREC_FP32)
+    pure = gpu.qureg(n)
+    channel_circuit(gpu, r32, pure, n, 2, seed=n)
+    gpu.destroy(pure)
+    for u in us:
+        gpu.multiQubitUnitary(r32, targets, u)
+    assert np.max(np.abs(gpu.state(r32) - got)) < 1e-5
+    gpu.destroy(r32)
+    gpu.destroy(regs[0])
+    cpuref.destroy(regs[1])
+
+
 @pytest.mark.parametrize("n,targets", [(11, [0, 1, 2, 3, 4]), (11, [6, 7, 8, 9, 10]), (12, [0, 3, 7, 10, 11]),
                                        (13, [4, 0, 12, 2, 1]), (14, [5, 6, 7, 8, 9]), (14, [13, 0, 9, 1, 12]),
                                        (10, [9, 0, 4, 2, 7]), (9, [0, 1, 2, 3, 4])])
