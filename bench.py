@@ -121,7 +121,35 @@ def run_fp64_per_gate(qe, n, steps, warmup, seed):
     return q, plans, timed, wall, ev_ms, cu
 
 
+def fused_bytes(qe, c, n, amp_bytes=8):
+    """Algorithmic HBM bytes of one application of a fused brick-circuit
+    period, and the number of full sweeps in it. A step of several gates
+    (padded to 5 qubits on registers of >= 9 qubits) or a lone 5-qubit gate
+    reads and writes the register once; a lone gate on fewer qubits runs its
+    per-gate kernel and moves what that moves: a one-qubit gate 2NB, a
+    controlledNot NB, the 3-control unitary 2NB/8."""
+    import ctypes as C
+    N = float(1 << n)
+    targs = (C.c_int * 64)()
+    total, sweeps = 0.0, 0
+    for s in range(qe.qmxCircuitNumBlocks(c)):
+        k = qe.qmxCircuitStep(c, s, targs, None, None)
+        if k >= 5 or k == 1:
+            total += 2 * N * amp_bytes
+            sweeps += k >= 5
+        elif k == 2:
+            total += N * amp_bytes
+        elif k == 4:
+            total += 2 * N * amp_bytes / 8
+        else:
+            total += 2 * N * amp_bytes
+    return total, sweeps
+
+
 def run_fp32_fused(qe, n, plans, warmup, steps, k):
+    """Fused periods on an n-qubit FP32 register: (register, wall time of the
+    timed periods, their steps, the FP64 norm^2 before renormalising, their
+    algorithmic bytes, their full 5-qubit sweeps)."""
     from quantumlab_amd import workloads as W
     from quantumlab_amd.quest import QMX_PREC_FP32
     q = qe.qureg(n, precision=QMX_PREC_FP32)
@@ -131,6 +159,8 @@ def run_fp32_fused(qe, n, plans, warmup, steps, k):
         qe.qmxCircuitFuse(c, k)
         circuits.append(c)
     blocks = sum(qe.qmxCircuitNumBlocks(c) for c in circuits[warmup:])
+    moved = [fused_bytes(qe, c, n) for c in circuits[warmup:]]
+    nbytes, sweeps = sum(m[0] for m in moved), sum(m[1] for m in moved)
     for s in range(warmup):
         qe.qmxCircuitApply(q, circuits[s], 1)
     qe.qmxSync()
@@ -142,8 +172,9 @@ def run_fp32_fused(qe, n, plans, warmup, steps, k):
     norm2 = qe.qmxRenormalise(q, 1)  # FP64 norm, periodic renormalisation
     for c in circuits:
         qe.qmxCircuitDestroy(c)
-    log(f"fp32 fused timed {steps} steps: wall {wall:.3f}s blocks {blocks} norm^2 before renorm {norm2:.12f}")
-    return q, wall, blocks, norm2
+    log(f"fp32 fused timed {steps} steps: wall {wall:.3f}s blocks {blocks} ({sweeps} full sweeps) "
+        f"norm^2 before renorm {norm2:.12f}")
+    return q, wall, blocks, norm2, nbytes, sweeps
 
 
 def run_fp32_large(qe, n, seed, k):
@@ -161,6 +192,7 @@ def run_fp32_large(qe, n, seed, k):
     c = W.build_circuit(qe, n, gates)
     qe.qmxCircuitFuse(c, k)
     blocks = qe.qmxCircuitNumBlocks(c)
+    nbytes, sweeps = fused_bytes(qe, c, n)
     qe.qmxSync()
     t0 = time.perf_counter()
     qe.qmxCircuitApply(q32, c, 1)
@@ -186,11 +218,11 @@ def run_fp32_large(qe, n, seed, k):
     qe.destroy(q32)
     wall = float(np.median(walls))
     N = 1 << n
-    gbs = blocks * 2.0 * N * 8 / wall / 1e9
+    gbs = nbytes / wall / 1e9
     log(f"fused fp32 at {n} qubits: {wall * 1e3:.1f} ms (median of {[round(w * 1e3, 1) for w in walls]}) for "
         f"{len(gates)} gates in {blocks} blocks ({gbs:.0f} GB/s); "
         f"fp64 per-gate {wall64:.2f} s; 1-F={1 - fid:.3e}, max|da|={maxabs:.3e}")
-    return {"qubits": n, "state_gib": N * 8 / 2**30, "gates": len(gates), "blocks": blocks,
+    return {"qubits": n, "state_gib": N * 8 / 2**30, "gates": len(gates), "blocks": blocks, "full_sweeps": sweeps,
             "ms_per_step": round(wall * 1e3, 3), "ms_per_step_each": [round(w * 1e3, 3) for w in walls],
             "gates_per_s": round(len(gates) / wall, 2),
             "fp64_per_gate_seconds": round(wall64, 3),
@@ -199,7 +231,7 @@ def run_fp32_large(qe, n, seed, k):
             "fidelity_error": float(1 - fid), "max_abs_amp_error": float(maxabs),
             "roofline": {"bound": "hbm", "kernel": "kFused5Tile", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                         "mfma_tflops": round(blocks * N * 192 / wall / 1e12, 2)}}
+                         "mfma_tflops": round(sweeps * N * 192 / wall / 1e12, 2)}}
 
 
 # CPUs this process may use, read before any OpenMP runtime starts (with
@@ -820,7 +852,8 @@ def main():
                   "config": {"workload": "state-vector leg skipped (--steps 0 or --only)", "qubits": n}}
 
     if not args.no_fused and q64 is not None and not getattr(args, "fused_large_only", False):
-        q32, wall32, blocks, norm2 = run_fp32_fused(qe, n, plans, args.warmup, args.steps, args.fuse_qubits)
+        q32, wall32, blocks, norm2, fbytes, fsweeps = run_fp32_fused(qe, n, plans, args.warmup, args.steps,
+                                                                     args.fuse_qubits)
         ip = qe.calcInnerProduct(q64, q32)
         # normalised fidelity: norms from FP64 reductions (the FP32-stored vector
         # re-rounds its norm^2 by ~1e-9 even right after renormalisation)
@@ -833,7 +866,8 @@ def main():
                                 "gates_per_s": round(len(timed) / wall32, 2),
                                 "fidelity_error": float(1 - fid), "max_abs_amp_error": float(maxabs),
                                 "max_prob_error": float(p_err), "norm2_before_renorm": float(norm2)}
-        fgbs = blocks * 2.0 * N * 8 / wall32 / 1e9
+        fgbs = fbytes / wall32 / 1e9
+        result["fused_fp32"]["full_sweeps_per_step"] = fsweeps / args.steps
         result["fused_fp32"]["roofline"] = {
             "bound": "hbm", "kernel": "kFused5Tile (5-qubit blocks on LDS tiles, v_mfma_f32_16x16x4f32, "
                                       "three real products per complex product)", "achieved": round(fgbs, 1),
@@ -841,9 +875,10 @@ def main():
             "traffic": profiled_traffic("fused32_fp32", "kFused5Tile", n)[0],
             "traffic_source": profiled_traffic("fused32_fp32", "kFused5Tile", n)[1],
             "bytes_per_block": 2.0 * N * 8,
-            "mfma_tflops": round(blocks * N * 192 / wall32 / 1e12, 2),
-            "note": "every block reads and writes the FP32 register once; 192 MFMA flops per amplitude; over the "
-                    "fused steps' wall time (launch gaps included)"}
+            "mfma_tflops": round(fsweeps * N * 192 / wall32 / 1e12, 2),
+            "note": "a block of several gates (padded to 5 qubits) reads and writes the FP32 register once, with "
+                    "192 MFMA flops per amplitude; a lone gate on fewer qubits runs its per-gate kernel and counts "
+                    "what that moves; over the fused steps' wall time (launch gaps included)"}
         log(f"fused fp32: {wall / wall32:.2f}x vs fp64 per-gate, 1-F={1 - fid:.3e}, max|da|={maxabs:.3e}")
         qe.destroy(q32)
     if q64 is not None:
@@ -855,11 +890,13 @@ def main():
         result["stream"] = run_stream(qe, n)
     if args.only == "fused-kernel":
         plans = [period_gates(n, s, args.seed) for s in range(args.warmup + args.fused_steps)]
-        q32, wall32, blocks, _ = run_fp32_fused(qe, n, plans, args.warmup, args.fused_steps, args.fuse_qubits)
+        q32, wall32, blocks, _, fbytes, fsweeps = run_fp32_fused(qe, n, plans, args.warmup, args.fused_steps,
+                                                                 args.fuse_qubits)
         qe.destroy(q32)
-        gbs = blocks * 2.0 * N * 8 / wall32 / 1e9
+        gbs = fbytes / wall32 / 1e9
         result["fused_kernel"] = {"qubits": n, "periods": args.fused_steps, "blocks": blocks,
-                                  "ms_per_block": round(wall32 / blocks * 1e3, 3), "GBps": round(gbs, 1)}
+                                  "full_sweeps": fsweeps, "ms_per_block": round(wall32 / blocks * 1e3, 3),
+                                  "GBps": round(gbs, 1)}
 
     if not args.no_density:
         result["density"] = run_density(qe, args.density_qubits, args.density_layers, args.seed)

@@ -437,47 +437,107 @@ double applyCircuit(const Qureg &q, qmxCircuit *c, int fused) {
     return prob;
 }
 
+// Fuse the gates of one segment (no barrier inside) into blocks of at most
+// maxQubits qubits. Gates on disjoint qubits commute, so any order that keeps
+// each qubit's gates in sequence is the same circuit. Each step takes the
+// window of maxQubits consecutive qubits (or a ready gate's own qubits when
+// they span more) that absorbs the most gates: from the window's qubits, the
+// next pending gate of a qubit joins while it is next on all of its qubits
+// and the block stays within maxQubits. On the 32-qubit brick periods this
+// makes 18-19 steps per period where placing each gate, in list order, into the
+// last block it fits made 26.
+void fuseSegment(qmxCircuit *c, const std::vector<int> &seg, int maxQubits) {
+    if (seg.empty()) return;
+    std::vector<std::vector<int>> per(64);
+    std::vector<uint64_t> maskOf(c->ops.size(), 0);
+    for (int gi : seg) {
+        maskOf[gi] = c->ops[gi].gate.qubitMask();
+        for (uint64_t m = maskOf[gi]; m; m &= m - 1) per[__builtin_ctzll(m)].push_back(gi);
+    }
+    std::vector<size_t> ptr(64, 0);
+    auto nextOn = [&](const std::vector<size_t> &p, int x) { return p[x] < per[x].size() ? per[x][p[x]] : -1; };
+    auto readyIn = [&](const std::vector<size_t> &p, int gi) {
+        for (uint64_t m = maskOf[gi]; m; m &= m - 1)
+            if (nextOn(p, __builtin_ctzll(m)) != gi) return false;
+        return true;
+    };
+    const uint64_t all = c->numQubits >= 64 ? ~0ull : (1ull << c->numQubits) - 1ull;
+    size_t remaining = seg.size();
+    std::vector<uint64_t> seeds;
+    while (remaining) {
+        seeds.clear();
+        int firstReady = -1;
+        for (int x = 0; x < 64; ++x) {
+            const int gi = nextOn(ptr, x);
+            if (gi < 0 || !readyIn(ptr, gi)) continue;
+            if (firstReady < 0 || gi < firstReady) firstReady = gi;
+            const uint64_t m = maskOf[gi];
+            const int lo = __builtin_ctzll(m), hi = 63 - __builtin_clzll(m);
+            if (hi - lo + 1 <= maxQubits) {
+                for (int k = std::max(0, hi - maxQubits + 1); k <= lo; ++k) {
+                    const uint64_t w = (maxQubits >= 64 ? ~0ull : ((1ull << maxQubits) - 1ull) << k) & all;
+                    seeds.push_back(w);
+                }
+            } else if (__builtin_popcountll(m) <= maxQubits) {
+                seeds.push_back(m);
+            }
+        }
+        std::sort(seeds.begin(), seeds.end());
+        seeds.erase(std::unique(seeds.begin(), seeds.end()), seeds.end());
+        Block best;
+        std::vector<size_t> bestPtr;
+        for (uint64_t w : seeds) {
+            std::vector<size_t> p = ptr;
+            Block b;
+            for (bool progress = true; progress;) {
+                progress = false;
+                for (uint64_t wm = w; wm; wm &= wm - 1) {
+                    const int x = __builtin_ctzll(wm);
+                    for (int gi = nextOn(p, x); gi >= 0; gi = nextOn(p, x)) {
+                        if (!readyIn(p, gi) || __builtin_popcountll(b.mask | maskOf[gi]) > maxQubits) break;
+                        b.mask |= maskOf[gi];
+                        b.gates.push_back(gi);
+                        for (uint64_t m = maskOf[gi]; m; m &= m - 1) ++p[__builtin_ctzll(m)];
+                        progress = true;
+                    }
+                }
+            }
+            if (b.gates.size() > best.gates.size()) {
+                best = std::move(b);
+                bestPtr = std::move(p);
+            }
+        }
+        if (best.gates.empty()) {
+            // a ready gate on more than maxQubits qubits: a block of its own
+            best.mask = maskOf[firstReady];
+            best.gates.push_back(firstReady);
+            bestPtr = ptr;
+            for (uint64_t m = best.mask; m; m &= m - 1) ++bestPtr[__builtin_ctzll(m)];
+        }
+        remaining -= best.gates.size();
+        ptr = std::move(bestPtr);
+        c->blocks.push_back(std::move(best));
+    }
+}
+
 int fuseCircuit(qmxCircuit *c, int maxQubits) {
     c->releaseBlocks();
     c->fusedQubits = maxQubits;
-    size_t segStart = 0;  // first block after the latest barrier
+    std::vector<int> seg;  // gates since the latest barrier
     for (int gi = 0; gi < (int)c->ops.size(); ++gi) {
         const Op &o = c->ops[gi];
         if (o.barrier) {
+            fuseSegment(c, seg, maxQubits);
+            seg.clear();
             Block nb;
             nb.barrierOp = gi;
             c->blocks.push_back(std::move(nb));
-            segStart = c->blocks.size();
             continue;
         }
         if (o.gate.kind == G_NONE) continue;  // noise or unsupported: nothing on a state vector
-        const uint64_t q = o.gate.qubitMask();
-        int last = -1;
-        for (int b = (int)c->blocks.size() - 1; b >= (int)segStart; --b)
-            if (c->blocks[b].mask & q) {
-                last = b;
-                break;
-            }
-        int best = -1, bestOverlap = -1;
-        for (int b = std::max((int)segStart, last); b < (int)c->blocks.size(); ++b) {
-            uint64_t u = c->blocks[b].mask | q;
-            if (__builtin_popcountll(u) > maxQubits) continue;
-            int ov = __builtin_popcountll(c->blocks[b].mask & q);
-            if (ov > bestOverlap) {
-                best = b;
-                bestOverlap = ov;
-            }
-        }
-        if (best < 0) {
-            Block nb;
-            nb.mask = q;
-            nb.gates.push_back(gi);
-            c->blocks.push_back(std::move(nb));
-        } else {
-            c->blocks[best].mask |= q;
-            c->blocks[best].gates.push_back(gi);
-        }
+        seg.push_back(gi);
     }
+    fuseSegment(c, seg, maxQubits);
     // The MFMA kernel applies exactly-5-qubit blocks: pad smaller multi-gate
     // blocks with identity on extra qubits, taken from the top so the low
     // qubits stay non-targets (consecutive groups = consecutive addresses).

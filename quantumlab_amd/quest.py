@@ -306,6 +306,7 @@ def _make_abi(qreal) -> types.SimpleNamespace:
                                                C.POINTER(qreal)]),
         "qmxCircuitFuse": (C.c_int, [C.c_void_p, C.c_int]),
         "qmxCircuitNumBlocks": (C.c_int, [C.c_void_p]),
+        "qmxCircuitStep": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(qreal), C.POINTER(qreal)]),
         "qmxCircuitApply": (qreal, [Qureg, C.c_void_p, C.c_int]),
         "qmxCircuitAddOperation": (None, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_int, qreal]),
         "qmxCircuitSetNoiseAfterGate": (None, [C.c_void_p, NoiseAfterGate]),
