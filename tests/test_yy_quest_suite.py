@@ -29,9 +29,17 @@ error against exact arithmetic is 9.1e-14 for the superoperator product and
 the draws. Every density operation is compared with QuEST's CPU backend at
 1e-12 on normalised states by the parity tests. Accepted here:
   f64  sections QuEST's own build fails, plus sections failing only by a
-       validation error on a matrix the suite drew at random or by density-
-       matrix comparisons at 1e-13 (state-vector comparisons, entries <= 1,
-       must pass);
+       validation error on a matrix the suite drew at random, by density-
+       matrix comparisons at 1e-13, or by setWeightedQureg's third state-
+       vector check (other state-vector comparisons, entries <= 1, must pass).
+       That check compares entries grown past 1e3 at 1e-12, below one ulp;
+       this library's weighted sum rounds exactly as QuEST's (pinned bit for
+       bit against QuEST's build by test_full_api.py), and QuEST's own sum
+       misses the check for 10 of 4000 random draws in a model of the
+       section. The draws in a run against this library are not those of
+       QuEST's own run, whose failures are the same every time: here the
+       random-matrix sections that fail have moved from run to run, and this
+       check failed in two of seven runs of the tag;
   f32  those, plus sections failing only by tolerance comparisons of any
        register (areEqual / Approx at 1e-5; QuEST's own float build fails 49
        sections besides the 4 distribution-limit ones), and the
@@ -102,6 +110,14 @@ def is_random_input_error(block):
     return "due to unexpected exception with message" in block and any(e in block for e in RANDOM_INPUT_ERRORS)
 
 
+def is_weighted_sum_rounding(block):
+    """setWeightedQureg's state-vector check after three weightings
+    (test_state_initialisations.cpp:588-592): entries past 1e3 compared at an
+    absolute 10 REAL_EPS, below one double ulp there, against the suite's sum in
+    another order"""
+    return block.strip().startswith("REQUIRE( areEqual(vecC, refOut, 10*")
+
+
 def is_random_map_left_valid(block):
     """a zeroed entry of a random Kraus map that left it trace-preserving
     within float's REAL_EPS: the expected error was not raised"""
@@ -109,7 +125,7 @@ def is_random_map_left_valid(block):
             and 'Contains("trace preserving")' in block)
 
 
-ACCEPTED_KINDS = {"f64": (is_random_input_error, is_density_comparison),
+ACCEPTED_KINDS = {"f64": (is_random_input_error, is_density_comparison, is_weighted_sum_rounding),
                   "f32": (is_random_input_error, is_comparison, is_random_map_left_valid)}
 
 
@@ -166,12 +182,16 @@ def test_failure_report_parser():
            "  false\n\n---\n"
            "/x/y/test_decoherence.cpp:564: FAILED:\n  REQUIRE_THROWS_WITH( mixKrausMap(qureg, 0, ops, numOps), "
            "Contains(\"trace preserving\") )\nbecause no exception was thrown where one was expected:\n\n---\n"
+           "/x/y/test_state_initialisations.cpp:592: FAILED:\n  REQUIRE( areEqual(vecC, refOut, 10*1e-13) )\n"
+           "with expansion:\n  false\n\n---\n"
            "/x/y/test_decoherence.cpp:1090: FAILED:\n  REQUIRE_THROWS_WITH( mixTwoQubitKrausMap(qureg, 0,1, ops, "
            "numOps), Contains(\"trace preserving\") )\nwith expansion:\n  \"Invalid target qubit.\" contains: "
            "\"trace preserving\"\n")
     b = failure_blocks(out)
     assert set(b) == {"test_unitaries.cpp:565", "test_unitaries.cpp:1779", "test_unitaries.cpp:560",
-                      "test_decoherence.cpp:564", "test_decoherence.cpp:1090"}
+                      "test_decoherence.cpp:564", "test_decoherence.cpp:1090", "test_state_initialisations.cpp:592"}
+    assert is_weighted_sum_rounding(b["test_state_initialisations.cpp:592"][0])
+    assert not is_weighted_sum_rounding(b["test_unitaries.cpp:560"][0])
     assert is_random_map_left_valid(b["test_decoherence.cpp:564"][0])
     assert is_density_comparison(b["test_unitaries.cpp:565"][0])
     assert is_comparison(b["test_unitaries.cpp:560"][0]) and not is_density_comparison(b["test_unitaries.cpp:560"][0])
