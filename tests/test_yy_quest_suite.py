@@ -39,7 +39,7 @@ the draws. Every density operation is compared with QuEST's CPU backend at
        section. The draws in a run against this library are not those of
        QuEST's own run, whose failures are the same every time: here the
        random-matrix sections that fail have moved from run to run, and this
-       check failed in two of seven runs of the tag;
+       check failed in one of five runs of the tag with the present sum;
   f32  those, plus sections failing only by tolerance comparisons of any
        register (areEqual / Approx at 1e-5; QuEST's own float build fails 49
        sections besides the 4 distribution-limit ones), and the
