@@ -1,0 +1,136 @@
+"""The circuit planner's fused blocks, on the CPU: qmxCircuitFuse groups a gate
+list into dense blocks of at most k qubits (qmx_circuit.cpp, fuseSegment), and
+qmxCircuitStep hands back each block's qubits and matrix. Applying the blocks'
+matrices in order must give the same operator as the gates one by one (the
+dense model of tests/dense.py, independent of the library); the 32-qubit
+brick period must fuse into at most 19 steps. Planning is host code: nothing
+here touches a GPU (the library is loaded without creating an environment)."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import quantumlab_amd
+import quantumlab_amd.quest as Q
+from quantumlab_amd import workloads as W
+from tests import dense as D
+
+
+class _Host:
+    """What workloads.build_circuit needs: the library and its FP64 ABI."""
+
+    def __init__(self):
+        self.lib = C.CDLL(quantumlab_amd.library_path(), mode=os.RTLD_LOCAL)
+        self.t = Q.abi(C.c_double)
+        self.real_ptr = C.POINTER(C.c_double)
+        for name, (res, args) in self.t.EXT_SIGS.items():
+            fn = getattr(self.lib, name, None)
+            if fn is not None:
+                fn.restype, fn.argtypes = res, args
+
+    @staticmethod
+    def reals(x):
+        return np.ascontiguousarray(x, dtype=np.float64)
+
+
+@pytest.fixture(scope="module")
+def host():
+    return _Host()
+
+
+def gate_op(n, g):
+    if g[0] == "cu":
+        a, b = g[2], g[3]
+        return D.full_op(n, [g[1]], np.array([[a, -np.conj(b)], [b, np.conj(a)]]))
+    if g[0] == "cnot":
+        return D.full_op(n, [g[2]], D.X, controls=[g[1]])
+    if g[0] == "rz":
+        return D.full_op(n, [g[1]], np.diag([np.exp(-0.5j * g[2]), np.exp(0.5j * g[2])]))
+    if g[0] == "mcu":
+        return D.full_op(n, [g[2]], g[3], controls=g[1])
+    if g[0] == "mqu":
+        return D.full_op(n, g[1], g[2])
+    raise ValueError(g[0])
+
+
+def steps_of(host, c, n):
+    """[(qubits, matrix)] of every fused step."""
+    out = []
+    targs = (C.c_int * 64)()
+    re, im = np.zeros(1 << 20), np.zeros(1 << 20)
+    for s in range(host.lib.qmxCircuitNumBlocks(c)):
+        k = host.lib.qmxCircuitStep(c, s, targs, re.ctypes.data_as(host.real_ptr), im.ctypes.data_as(host.real_ptr))
+        dim = 1 << k
+        out.append((list(targs[:k]), (re[:dim * dim] + 1j * im[:dim * dim]).reshape(dim, dim)))
+    return out
+
+
+def random_circuit(n, count, seed):
+    """One- and two-qubit gates anywhere, plus wide controlled and 5-target gates."""
+    rng = np.random.default_rng(seed)
+    gates = []
+    for _ in range(count):
+        r = rng.random()
+        if r < 0.45:
+            a = np.exp(1j * rng.uniform(0, 2 * np.pi)) * np.sqrt(rng.random())
+            b = np.exp(1j * rng.uniform(0, 2 * np.pi)) * np.sqrt(1 - abs(a) ** 2)
+            gates.append(("cu", int(rng.integers(n)), a, b))
+        elif r < 0.75:
+            c, t = rng.choice(n, 2, replace=False)
+            gates.append(("cnot", int(c), int(t)))
+        elif r < 0.85:
+            gates.append(("rz", int(rng.integers(n)), float(rng.uniform(0, 2 * np.pi))))
+        elif r < 0.93:
+            qs = [int(x) for x in rng.choice(n, 4, replace=False)]
+            gates.append(("mcu", qs[:3], qs[3], W.haar_unitary(rng, 2)))
+        else:
+            gates.append(("mqu", [int(x) for x in rng.choice(n, 5, replace=False)], W.haar_unitary(rng, 32)))
+    return gates
+
+
+@pytest.mark.parametrize("n,k,kind,seed", [(8, 5, "brick", 12), (10, 5, "brick", 3), (10, 3, "brick", 4),
+                                           (9, 4, "random", 5), (10, 5, "random", 6), (7, 2, "random", 7),
+                                           (11, 5, "random", 8)])
+def test_fused_steps_reproduce_the_gate_list(host, n, k, kind, seed):
+    gates = W.brick_circuit(n, 10, seed=seed, extra_every=5) if kind == "brick" else random_circuit(n, 120, seed)
+    c = W.build_circuit(host, n, gates)
+    try:
+        nsteps = host.lib.qmxCircuitFuse(c, k)
+        steps = steps_of(host, c, n)
+        assert nsteps == len(steps) and 0 < nsteps < len(gates)
+        widest = max(len(gate_qubits(g)) for g in gates)
+        assert all(len(qs) <= max(k, widest) for qs, _ in steps)  # only a lone gate exceeds k
+        rng = np.random.default_rng(seed + 100)
+        psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+        want = psi.copy()
+        for g in gates:
+            want = gate_op(n, g) @ want
+        got = psi.copy()
+        for qs, u in steps:
+            got = D.full_op(n, qs, u) @ got
+        assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want))
+    finally:
+        host.lib.qmxCircuitDestroy(c)
+
+
+def gate_qubits(g):
+    if g[0] in ("cu", "rz"):
+        return [g[1]]
+    if g[0] == "cnot":
+        return [g[1], g[2]]
+    if g[0] == "mcu":
+        return list(g[1]) + [g[2]]
+    return list(g[1])
+
+
+def test_brick_period_step_count(host):
+    """Each 5-layer period of the 32-qubit brick circuit (one 3-control and one
+    5-target gate among 244) fuses into at most 19 steps of <= 5 qubits."""
+    for s in range(3):
+        c = W.build_circuit(host, 32, W.brick_circuit(32, 5, 2024, extra_every=5, first_layer=5 * s))
+        try:
+            assert host.lib.qmxCircuitFuse(c, 5) <= 19
+            assert all(len(qs) <= 5 for qs, _ in steps_of(host, c, 32))
+        finally:
+            host.lib.qmxCircuitDestroy(c)
