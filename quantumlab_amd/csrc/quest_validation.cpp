@@ -442,8 +442,16 @@ void oneQubitPauliProbs(double px, double py, double pz, const char *f) {
     check(pz <= none, E_INVALID_ONE_QUBIT_PAULI_PROBS, f);
 }
 bool validPauliCode(int code) { return code == PAULI_I || code == PAULI_X || code == PAULI_Y || code == PAULI_Z; }
+// A C caller's code array may hold any int. Each element is read as its int
+// storage: loading an out-of-range value through the C++ enum type is undefined,
+// and would let the compiler assume the very range this check tests.
+static_assert(sizeof(enum pauliOpType) == sizeof(int), "pauliOpType is stored as an int");
 void pauliCodes(const enum pauliOpType *codes, int n, const char *f) {
-    for (int i = 0; i < n; ++i) check(validPauliCode((int)codes[i]), E_INVALID_PAULI_CODE, f);
+    for (int i = 0; i < n; ++i) {
+        int code;
+        std::memcpy(&code, codes + i, sizeof code);
+        check(validPauliCode(code), E_INVALID_PAULI_CODE, f);
+    }
 }
 void numPauliSumTerms(int n, const char *f) { check(n > 0, E_INVALID_NUM_SUM_TERMS, f); }
 
