@@ -51,7 +51,7 @@ extern "C" {
 
 // QuEST-style PauliHamil -> the planner's 16 inputs (see the header for the list)
 int qmxTrotterFeatures(PauliHamil h, qreal time, float *out) {
-    if (h.numQubits < 1 || h.numQubits > 64 || h.numSumTerms < 1 || !out) {
+    if (h.numQubits < 1 || h.numQubits > 64 || h.numSumTerms < 1 || !h.pauliCodes || !h.termCoeffs || !out) {
         std::fprintf(stderr, "libquest_mi355x: qmxTrotterFeatures: invalid Hamiltonian\n");
         return 0;
     }
@@ -60,7 +60,12 @@ int qmxTrotterFeatures(PauliHamil h, qreal time, float *out) {
     for (int t = 0; t < h.numSumTerms; ++t) {
         Term tm{0, 0, 0, h.termCoeffs[t]};
         for (int q = 0; q < n; ++q) {
-            const int c = h.pauliCodes[(size_t)t * n + q];
+            int c;  // the int storage: a C caller may have written any value
+            std::memcpy(&c, h.pauliCodes + (size_t)t * n + q, sizeof c);
+            if (c < PAULI_I || c > PAULI_Z) {
+                std::fprintf(stderr, "libquest_mi355x: qmxTrotterFeatures: invalid Pauli code %d\n", c);
+                return 0;
+            }
             const uint64_t b = 1ull << q;
             if (c == PAULI_X) tm.x |= b;
             if (c == PAULI_Y) { tm.x |= b; tm.z |= b; }

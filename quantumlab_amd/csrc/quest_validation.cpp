@@ -444,14 +444,15 @@ void oneQubitPauliProbs(double px, double py, double pz, const char *f) {
 bool validPauliCode(int code) { return code == PAULI_I || code == PAULI_X || code == PAULI_Y || code == PAULI_Z; }
 // A C caller's code array may hold any int. Each element is read as its int
 // storage: loading an out-of-range value through the C++ enum type is undefined,
-// and would let the compiler assume the very range this check tests.
+// and would let the compiler assume the very range the checks test.
 static_assert(sizeof(enum pauliOpType) == sizeof(int), "pauliOpType is stored as an int");
+int pauliCodeAt(const enum pauliOpType *codes, long long i) {
+    int code;
+    std::memcpy(&code, codes + i, sizeof code);
+    return code;
+}
 void pauliCodes(const enum pauliOpType *codes, int n, const char *f) {
-    for (int i = 0; i < n; ++i) {
-        int code;
-        std::memcpy(&code, codes + i, sizeof code);
-        check(validPauliCode(code), E_INVALID_PAULI_CODE, f);
-    }
+    for (int i = 0; i < n; ++i) check(validPauliCode(pauliCodeAt(codes, i)), E_INVALID_PAULI_CODE, f);
 }
 void numPauliSumTerms(int n, const char *f) { check(n > 0, E_INVALID_NUM_SUM_TERMS, f); }
 
@@ -533,7 +534,7 @@ void diagonalOp(const Qureg &q, const DiagonalOp &op, const char *f) {
 }
 bool diagonalHamil(const PauliHamil &h) {
     for (long long p = 0; p < (long long)h.numSumTerms * h.numQubits; ++p)
-        if (h.pauliCodes[p] != PAULI_I && h.pauliCodes[p] != PAULI_Z) return false;
+        if (pauliCodeAt(h.pauliCodes, p) != PAULI_I && pauliCodeAt(h.pauliCodes, p) != PAULI_Z) return false;
     return true;
 }
 void diagPauliHamil(const DiagonalOp &op, const PauliHamil &h, const char *f) {

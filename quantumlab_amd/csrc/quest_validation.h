@@ -151,6 +151,8 @@ void oneQubitDepolProb(double p, const char *f);
 void oneQubitDampingProb(double p, const char *f);
 void twoQubitDepolProb(double p, const char *f);
 void oneQubitPauliProbs(double px, double py, double pz, const char *f);
+// element i of a caller's Pauli-code array, read as its int storage
+int pauliCodeAt(const enum pauliOpType *codes, long long i);
 void pauliCodes(const enum pauliOpType *codes, int n, const char *f);
 void numPauliSumTerms(int n, const char *f);
 void oneQubitKrausMapDims(const Qureg &q, int numOps, const char *f);

@@ -9,7 +9,8 @@
  *                    with the sanitizers, its .hip objects as built; also at
  *                    QuEST_PREC=1 as host_api_lib_f32): what runs
  *                    without a device — circuit recording and fusion, PauliHamil
- *                    files, matrices, seeding — and the rejection of bad input
+ *                    files, matrices, the planner's inputs, seeding — and the
+ *                    rejection of bad input
  *                    through a returning error hook.
  * The pass condition is exit status 0 with no sanitizer report. */
 #define _POSIX_C_SOURCE 200809L  /* mkdtemp */
@@ -219,6 +220,21 @@ static void matrices(void) {
     EXPECT_ERROR(createComplexMatrixN(0));
 }
 
+/* the Trotter planner's inputs from a Hamiltonian, and a code array holding a
+ * value outside 0..3, which it refuses */
+static void plannerFeatures(void) {
+    PauliHamil h;
+    EXPECT_OK(h = createPauliHamil(6, 4));
+    for (int t = 0; t < 4; ++t) h.termCoeffs[t] = (qreal)(0.3 * (t + 1));
+    for (int i = 0; i < 24; ++i) h.pauliCodes[i] = (enum pauliOpType)((i * 5 + 2) % 4);
+    float f[16];
+    CHECK(qmxTrotterFeatures(h, (qreal)0.5, f) == 1);
+    for (int i = 0; i < 16; ++i) CHECK(isfinite(f[i]));
+    h.pauliCodes[7] = (enum pauliOpType)7;
+    CHECK(qmxTrotterFeatures(h, (qreal)0.5, f) == 0);
+    destroyPauliHamil(h);
+}
+
 /* QuEST's seed array (QuEST.c:1597-1626) and the generator state it feeds */
 static void seeding(void) {
     QuESTEnv env;
@@ -243,6 +259,7 @@ int main(void) {
     circuits();
     hamilFiles(dir);
     matrices();
+    plannerFeatures();
     seeding();
     rmdir(dir);
     printf("host_api (library host code, qreal of %d bytes): %d rejections reported, all expected\n",
