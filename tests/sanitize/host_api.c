@@ -63,6 +63,32 @@ static ComplexMatrix2 hadamardMatrix(void) {
     return u;
 }
 
+/* fusion at every block width, each step read back in full and with NULL outputs */
+static void fuseAtEveryWidth(qmxCircuit *c, int n, int recorded) {
+    static qreal bre[1 << 20], bim[1 << 20];
+    int targs[64];
+    for (int k = 1; k <= 10; ++k) {
+        int steps = 0;
+        EXPECT_OK(steps = qmxCircuitFuse(c, k));
+        CHECK(steps > 0 && steps <= recorded);
+        for (int s = 0; s < steps; ++s) {
+            const int w = qmxCircuitStep(c, s, targs, bre, bim);
+            CHECK(w >= 0 && w <= (k > 5 ? k : 5) && w <= n);
+            for (int i = 1; i < w; ++i) CHECK(targs[i - 1] < targs[i]);
+            for (int i = 0; i < w; ++i) CHECK(targs[i] >= 0 && targs[i] < n);
+            CHECK(qmxCircuitStep(c, s, NULL, NULL, NULL) == w);
+            if (w > 0) {  /* each block is unitary: its first column has unit norm */
+                const int dim = 1 << w;
+                double norm = 0;
+                for (int r = 0; r < dim; ++r) norm += bre[r * dim] * bre[r * dim] + bim[r * dim] * bim[r * dim];
+                CHECK(fabs(norm - 1) < (sizeof(qreal) == 8 ? 1e-12 : 1e-5));
+            }
+        }
+        CHECK(qmxCircuitStep(c, steps, targs, bre, bim) == -1);
+        CHECK(qmxCircuitNumBlocks(c) <= steps);
+    }
+}
+
 /* every operation code, the typed recorders, fusion at every width, and the
  * records a circuit must refuse */
 static void circuits(void) {
@@ -123,29 +149,42 @@ static void circuits(void) {
     EXPECT_ERROR(none = qmxCircuitCreate(0));
     CHECK(none == NULL);
 
-    /* fusion at every width, each step read back in full and with NULL outputs */
-    static qreal bre[1 << 20], bim[1 << 20];
-    int targs[64];
-    for (int k = 1; k <= 10; ++k) {
-        int steps = 0;
-        EXPECT_OK(steps = qmxCircuitFuse(c, k));
-        CHECK(steps > 0 && steps <= recorded);
-        for (int s = 0; s < steps; ++s) {
-            const int w = qmxCircuitStep(c, s, targs, bre, bim);
-            CHECK(w >= 0 && w <= (k > 5 ? k : 5));
-            for (int i = 1; i < w; ++i) CHECK(targs[i - 1] < targs[i]);
-            CHECK(qmxCircuitStep(c, s, NULL, NULL, NULL) == w);
-            if (w > 0) {  /* each block is unitary: its first column has unit norm */
-                const int dim = 1 << w;
-                double norm = 0;
-                for (int r = 0; r < dim; ++r) norm += bre[r * dim] * bre[r * dim] + bim[r * dim] * bim[r * dim];
-                CHECK(fabs(norm - 1) < (sizeof(qreal) == 8 ? 1e-12 : 1e-5));
-            }
-        }
-        CHECK(qmxCircuitStep(c, steps, targs, bre, bim) == -1);
-        CHECK(qmxCircuitNumBlocks(c) <= steps);
-    }
+    fuseAtEveryWidth(c, n, recorded);
     qmxCircuitDestroy(c);
+}
+
+/* circuits narrower than a block (padding cannot reach five qubits) and wider
+ * than 32 qubits (64-bit masks), fused at every width */
+static void circuitWidths(void) {
+    static const int widths[] = {1, 2, 3, 5, 6, 33, 40};
+    qreal re[32 * 32], im[32 * 32];
+    memset(re, 0, sizeof re);
+    memset(im, 0, sizeof im);
+    for (int i = 0; i < 32; ++i) re[((i * 7 + 1) % 32) * 32 + i] = 1;
+    for (size_t w = 0; w < sizeof widths / sizeof widths[0]; ++w) {
+        const int n = widths[w];
+        qmxCircuit *c = qmxCircuitCreate(n);
+        CHECK(c != NULL);
+        for (int i = 0; i < 40; ++i) {
+            const int t = (i * 5) % n, u = (t + 1 + i % 3) % n;
+            int qs[2] = {t, u};
+            EXPECT_OK(qmxCircuitAddOperation(c, QMX_OP_H, qs, 1, 0));
+            EXPECT_OK(qmxCircuitRotateZ(c, t, (qreal)(0.1 * i)));
+            if (n > 1 && u != t) EXPECT_OK(qmxCircuitAddOperation(c, QMX_OP_CRY, qs, 2, (qreal)0.2));
+            if (n >= 4 && i % 7 == 0) {
+                int ctrls[3] = {(t + 1) % n, (t + 2) % n, (t + 3) % n};
+                EXPECT_OK(qmxCircuitMultiControlledUnitary(c, ctrls, 3, t, hadamardMatrix()));
+            }
+            if (n >= 5 && i % 11 == 0) {
+                int targs5[5] = {0, n / 4, n / 2, (3 * n) / 4, n - 1};
+                if (n == 5) targs5[1] = 1, targs5[2] = 2, targs5[3] = 3;
+                EXPECT_OK(qmxCircuitMultiQubitUnitary(c, targs5, 5, re, im));
+            }
+            if (i == 20) EXPECT_OK(qmxCircuitAddOperation(c, QMX_OP_CC, qs, 1, 1));
+        }
+        fuseAtEveryWidth(c, n, qmxCircuitNumGates(c));
+        qmxCircuitDestroy(c);
+    }
 }
 
 static void writeFile(const char *path, const char *text) {
@@ -257,6 +296,7 @@ int main(void) {
     snprintf(dir, sizeof dir, "%s/host_api_XXXXXX", tmp && *tmp ? tmp : "/tmp");
     CHECK(mkdtemp(dir) != NULL);
     circuits();
+    circuitWidths();
     hamilFiles(dir);
     matrices();
     plannerFeatures();
