@@ -67,7 +67,8 @@ def steps_of(host, c, n):
 
 
 def random_circuit(n, count, seed):
-    """One- and two-qubit gates anywhere, plus wide controlled and 5-target gates."""
+    """One- and two-qubit gates anywhere, plus wide controlled and 5-target gates
+    (on registers of at least five qubits)."""
     rng = np.random.default_rng(seed)
     gates = []
     for _ in range(count):
@@ -76,10 +77,10 @@ def random_circuit(n, count, seed):
             a = np.exp(1j * rng.uniform(0, 2 * np.pi)) * np.sqrt(rng.random())
             b = np.exp(1j * rng.uniform(0, 2 * np.pi)) * np.sqrt(1 - abs(a) ** 2)
             gates.append(("cu", int(rng.integers(n)), a, b))
-        elif r < 0.75:
+        elif r < 0.75 and n >= 2:
             c, t = rng.choice(n, 2, replace=False)
             gates.append(("cnot", int(c), int(t)))
-        elif r < 0.85:
+        elif r < 0.85 or n < 5:
             gates.append(("rz", int(rng.integers(n)), float(rng.uniform(0, 2 * np.pi))))
         elif r < 0.93:
             qs = [int(x) for x in rng.choice(n, 4, replace=False)]
@@ -91,7 +92,8 @@ def random_circuit(n, count, seed):
 
 @pytest.mark.parametrize("n,k,kind,seed", [(8, 5, "brick", 12), (10, 5, "brick", 3), (10, 3, "brick", 4),
                                            (9, 4, "random", 5), (10, 5, "random", 6), (7, 2, "random", 7),
-                                           (11, 5, "random", 8)])
+                                           (11, 5, "random", 8), (1, 5, "random", 9), (2, 5, "random", 10),
+                                           (3, 5, "random", 11), (3, 2, "random", 12), (5, 5, "random", 13)])
 def test_fused_steps_reproduce_the_gate_list(host, n, k, kind, seed):
     gates = W.brick_circuit(n, 10, seed=seed, extra_every=5) if kind == "brick" else random_circuit(n, 120, seed)
     c = W.build_circuit(host, n, gates)
