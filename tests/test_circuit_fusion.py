@@ -2,8 +2,9 @@
 list into dense blocks of at most k qubits (qmx_circuit.cpp, fuseSegment), and
 qmxCircuitStep hands back each block's qubits and matrix. Applying the blocks'
 matrices in order must give the same operator as the gates one by one (the
-dense model of tests/dense.py, independent of the library); the 32-qubit
-brick period must fuse into at most 19 steps. Planning is host code: nothing
+dense model of tests/dense.py, independent of the library), on registers of 1
+to 11 qubits and, restricted to the qubits touched, on 33- to 50-qubit
+registers; the 32-qubit brick period must fuse into at most 19 steps. Planning is host code: nothing
 here touches a GPU (the library is loaded without creating an environment)."""
 import ctypes as C
 import os
@@ -136,3 +137,71 @@ def test_brick_period_step_count(host):
             assert all(len(qs) <= 5 for qs, _ in steps_of(host, c, 32))
         finally:
             host.lib.qmxCircuitDestroy(c)
+
+
+def apply_local(psi, m, qs, u):
+    """u (qs[0] = least significant bit of its index) on qubits qs of an m-qubit vector."""
+    k = len(qs)
+    axes = [m - 1 - q for q in reversed(qs)]  # C order: the last axis is bit 0
+    t = np.tensordot(u.reshape([2] * (2 * k)), psi.reshape([2] * m), axes=(list(range(k, 2 * k)), axes))
+    return np.moveaxis(t, list(range(k)), axes).reshape(-1)
+
+
+def local_gate(g):
+    """(qubits, matrix) of a recorded gate, its targets first, then its controls."""
+    if g[0] == "cu":
+        return [g[1]], np.array([[g[2], -np.conj(g[3])], [g[3], np.conj(g[2])]])
+    if g[0] == "rz":
+        return [g[1]], np.diag([np.exp(-0.5j * g[2]), np.exp(0.5j * g[2])])
+    if g[0] == "cnot":
+        return [g[2], g[1]], D.full_op(2, [0], D.X, controls=[1])
+    if g[0] == "mcu":
+        k = 1 + len(g[1])
+        return [g[2]] + list(g[1]), D.full_op(k, [0], g[3], controls=list(range(1, k)))
+    return list(g[1]), g[2]
+
+
+@pytest.mark.parametrize("n,seed", [(33, 21), (40, 22), (50, 23)])
+def test_wide_register_steps_reproduce_the_gate_list(host, n, seed):
+    """Registers past 32 qubits (64-bit qubit masks, padding from the top qubit
+    down): gates on 12 qubits spread over the whole register, six of them at 26
+    or above, the top qubit among them. The fused steps, applied to the qubits they name, reproduce the
+    gate list on the vector of every qubit either one touches."""
+    rng = np.random.default_rng(seed)
+    high = [int(q) for q in rng.choice(np.arange(26, n - 1), 5, replace=False)] + [n - 1]
+    live = sorted(high) + list(range(0, 30, 5))
+    gates = [(g[0],) + tuple(_relabel(g[1:], live)) for g in random_circuit(12, 150, seed)]
+    c = W.build_circuit(host, n, gates)
+    try:
+        nsteps = host.lib.qmxCircuitFuse(c, 5)
+        steps = steps_of(host, c, n)
+        assert nsteps == len(steps) and 0 < nsteps < len(gates)
+        assert all(len(qs) == len(set(qs)) and all(0 <= q < n for q in qs) for qs, _ in steps)
+        used = sorted(set(live) | {q for qs, _ in steps for q in qs})
+        assert len(used) <= 20
+        pos = {q: i for i, q in enumerate(used)}
+        m = len(used)
+        psi = rng.standard_normal(1 << m) + 1j * rng.standard_normal(1 << m)
+        want = psi.copy()
+        for g in gates:
+            qs, u = local_gate(g)
+            want = apply_local(want, m, [pos[q] for q in qs], u)
+        got = psi.copy()
+        for qs, u in steps:
+            got = apply_local(got, m, [pos[q] for q in qs], u)
+        assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want))
+    finally:
+        host.lib.qmxCircuitDestroy(c)
+
+
+def _relabel(fields, live):
+    """a gate of random_circuit(12, ...) with qubit i renamed live[i]"""
+    out = []
+    for f in fields:
+        if isinstance(f, (int, np.integer)) and not isinstance(f, bool):
+            out.append(live[f])
+        elif isinstance(f, list):
+            out.append([live[q] for q in f])
+        else:
+            out.append(f)
+    return out
