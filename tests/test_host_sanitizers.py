@@ -42,6 +42,8 @@ def test_host_code_runs_clean_under_sanitizers(drivers, tmp_path, name, expect):
     r = subprocess.run([os.path.join(drivers, name)], capture_output=True, text=True, timeout=300, cwd=tmp_path,
                        env=env)
     out = r.stdout + r.stderr
+    if "does not come first in initial library list" in out:
+        pytest.skip("AddressSanitizer cannot start in this process environment (another preloaded library)")
     assert r.returncode == 0, out[-4000:]
     assert "AddressSanitizer" not in out and "runtime error" not in out and "LeakSanitizer" not in out, out[-4000:]
     assert expect in out
