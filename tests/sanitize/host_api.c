@@ -4,7 +4,8 @@
  *   host_api_cpuref  linked to the oracle's C restatement (oracle/cpuref/cpuref.c):
  *                    every gate, channel, reduction, measurement and state
  *                    initialisation the oracle implements, on a 5-qubit state
- *                    vector and a 3-qubit density register;
+ *                    vector and a 3-qubit density register, and its generator
+ *                    and swap-test closed form;
  *   host_api_lib     linked to libquest_mi355x's host code (its .cpp files built
  *                    with the sanitizers, its .hip objects as built; also at
  *                    QuEST_PREC=1 as host_api_lib_f32): what runs
@@ -458,7 +459,34 @@ static void trotter(Qureg q) {
     destroyPauliHamil(h);
 }
 
+/* the oracle's extra entry points: its generator and the swap-test closed form
+ * (ragged batch: registers of 1, 3 and 2 qubits) */
+void cpurefMtInitByArray(const unsigned long *key, int len);
+unsigned int cpurefMtNextU32(void);
+int cpurefNumThreads(void);
+void cpurefSwapP0ClosedForm(int numTests, const int *q, const double *a, const long long *aOff, const double *b,
+                            const long long *bOff, double *p0);
+
+static void extras(void) {
+    unsigned long key[3] = {1, 2, 3};
+    cpurefMtInitByArray(key, 3);
+    unsigned int x = 0;
+    for (int i = 0; i < 1000; ++i) x ^= cpurefMtNextU32();
+    (void)x;
+    CHECK(cpurefNumThreads() >= 1);
+    const int q[3] = {1, 3, 2};
+    const long long off[3] = {0, 2, 10};
+    double a[14], b[14], p0[3];
+    for (int i = 0; i < 14; ++i) a[i] = b[i] = 0;
+    a[0] = b[1] = 1;                    /* orthogonal: P0 = 1/2 */
+    for (int i = 2; i < 10; ++i) a[i] = b[i] = sqrt(1.0 / 8);  /* equal: P0 = 1 */
+    a[10] = b[10] = 1;                  /* equal basis states: P0 = 1 */
+    cpurefSwapP0ClosedForm(3, q, a, off, b, off, p0);
+    CHECK(fabs(p0[0] - 0.5) < 1e-15 && fabs(p0[1] - 1) < 1e-12 && fabs(p0[2] - 1) < 1e-15);
+}
+
 int main(void) {
+    extras();
     QuESTEnv env = createQuESTEnv();
     unsigned long keys[2] = {1234, 5678};
     seedQuEST(&env, keys, 2);
