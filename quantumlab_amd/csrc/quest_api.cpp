@@ -669,7 +669,9 @@ void destroyPauliHamil(PauliHamil h) {
 //  - terms: the non-empty lines ('\n'-separated; a line of blanks counts);
 //  - values: the whole file is one stream of numbers separated by any white
 //    space, read in order (coefficient, then n codes, per term), each number
-//    in C's strtod / strtol syntax; the first that does not parse is an error.
+//    as fscanf's "%lf" / "%d" take it (the longest prefix of a number; glibc
+//    ignores a dangling exponent such as "1e"); the first that does not
+//    convert is an error.
 namespace {
 
 struct HamilText {
@@ -678,14 +680,48 @@ struct HamilText {
     void skipSpace() {
         while (at < text.size() && std::isspace((unsigned char)text[at])) ++at;
     }
+    // The input item of a "%lf" / "%f" conversion as glibc's fscanf takes it:
+    // the longest prefix of a number (C11 7.21.6.2p9), so a dangling exponent
+    // ("1e", "2.5E-", "0x1p") is read and ignored; it converts when the item
+    // holds a mantissa digit or is inf / infinity / nan (no "(chars)" suffix).
+    // Returns the item's length, 0 when the conversion fails.
+    static size_t realItem(const char *s) {
+        auto ci = [](const char *a, const char *word) {
+            for (; *word; ++a, ++word)
+                if (std::tolower((unsigned char)*a) != *word) return false;
+            return true;
+        };
+        auto digits = [](const char *a, size_t i, bool hex, size_t *count) {
+            while (hex ? std::isxdigit((unsigned char)a[i]) : std::isdigit((unsigned char)a[i])) ++i, ++*count;
+            return i;
+        };
+        size_t i = (s[0] == '+' || s[0] == '-') ? 1 : 0;
+        if (ci(s + i, "infinity")) return i + 8;
+        if (ci(s + i, "inf") || ci(s + i, "nan")) return i + 3;
+        const bool hex = s[i] == '0' && (s[i + 1] == 'x' || s[i + 1] == 'X');
+        if (hex) i += 2;
+        size_t mantissa = 0;
+        i = digits(s, i, hex, &mantissa);
+        if (s[i] == '.') i = digits(s, i + 1, hex, &mantissa);
+        if (mantissa == 0) return 0;
+        const char e = (char)std::tolower((unsigned char)s[i]);
+        if (e == (hex ? 'p' : 'e')) {
+            ++i;
+            if (s[i] == '+' || s[i] == '-') ++i;
+            size_t expDigits = 0;
+            i = digits(s, i, false, &expDigits);
+        }
+        return i;
+    }
     bool real(qreal *out) {
         skipSpace();
         const char *begin = text.c_str() + at;
-        char *end = nullptr;
-        const qreal v = sizeof(qreal) == sizeof(float) ? (qreal)std::strtof(begin, &end) : (qreal)std::strtod(begin, &end);
-        if (end == begin) return false;
-        at += (size_t)(end - begin);
-        *out = v;
+        const size_t len = realItem(begin);
+        if (len == 0) return false;
+        const std::string item(begin, len);  // strtod of the item ignores a dangling exponent
+        *out = sizeof(qreal) == sizeof(float) ? (qreal)std::strtof(item.c_str(), nullptr)
+                                               : (qreal)std::strtod(item.c_str(), nullptr);
+        at += len;
         return true;
     }
     bool integer(int *out) {

@@ -37,7 +37,39 @@ HAMIL_FILES = [
     ("empty", ""),
     ("only_coefficient", "0.5\n"),
     ("code_with_suffix", "0.5 1x 2\n"),
+    ("dangling_exponent", "1e 1 2\n2.5E+ 3 0\n-1.e- 0 1\n"),
+    ("hex_coefficients", "0x1p-2 1 2\n-0X1.8P1 3 0\n0x1p 0 1\n"),
+    ("inf_and_nan_names", "inf 1\n-Infinity 2\nNAN 3\n"),
+    ("nan_with_chars", "nan(7) 0\n"),
+    ("hex_without_digits", "0x 1 2\n"),
+    ("lone_sign", "- 1 2\n"),
+    ("lone_point", ". 1 2\n"),
+    ("exponent_then_letter", "1e+x 1 2\n"),
 ]
+
+
+def _same(a, b):
+    """equal reads, a NaN coefficient matching a NaN"""
+    def norm(r):
+        return {k: ([("nan" if x != x else x) for x in v] if isinstance(v, list) else v) for k, v in r.items()}
+    return norm(a) == norm(b)
+
+
+def _random_file(rng):
+    """A PauliHamil file from the format's tokens and near-misses: numbers in
+    every fscanf syntax, out-of-range and malformed codes, ragged terms, tabs,
+    CRLF, blank lines, leading blanks, no final newline."""
+    coef = ["0.5", "-1.25", "1e-3", "+2", "3.", ".5", "-0", "abc", "1e", "0x1p3", "inf", "nan", "1x", "7", "2.5E+"]
+    code = ["4", "-1", "01", "3.0", "x", "+1", "2e0"]
+    width, lines = rng.integers(1, 5), []
+    for _ in range(rng.integers(1, 5)):
+        toks = [str(rng.choice(coef)) if rng.random() < 0.3 else str(rng.choice(["0.5", "-1.25", "0.125"]))]
+        k = width + (int(rng.choice([-1, 1])) if rng.random() < 0.1 else 0)
+        toks += [str(rng.choice(code)) if rng.random() < 0.1 else str(rng.integers(0, 4)) for _ in range(max(0, k))]
+        line = toks[0] + "".join(str(rng.choice([" ", " ", "  ", "\t"])) + t for t in toks[1:])
+        lines.append((" " if rng.random() < 0.1 else "") + line)
+    text = "".join(l + str(rng.choice(["\n", "\n", "\r\n", "\n\n", " \n"])) for l in lines)
+    return text.rstrip("\n") if rng.random() < 0.2 else text
 
 _CHILD = r"""
 import ctypes as C, json, sys
@@ -93,7 +125,24 @@ def test_pauli_hamil_file_matches_reference_reader(tmp_path, name, text):
         f.write(text)
     ref = reference_read(path)
     got = library_read(path)
-    assert got == ref
+    assert _same(got, ref), (ref, got)
+
+
+def test_random_pauli_hamil_files_match_reference_reader(tmp_path):
+    """80 seeded random files: the library reads each as QuEST's CPU build does
+    (same values, or the same message)."""
+    if not os.path.exists(REFLIB):
+        pytest.skip("reference QuEST build (oracle/_ref) not present")
+    import numpy as np
+
+    rng = np.random.default_rng(2024)
+    for i in range(80):
+        text = _random_file(rng)
+        path = str(tmp_path / f"h{i}.txt")
+        with open(path, "w", newline="") as f:
+            f.write(text)
+        ref, got = reference_read(path), library_read(path)
+        assert _same(got, ref), (repr(text), ref, got)
 
 
 def test_pauli_hamil_file_known_answer(tmp_path):
