@@ -74,8 +74,9 @@ def _random_file(rng):
 _CHILD = r"""
 import ctypes as C, json, sys
 lib = C.CDLL(sys.argv[1])
+qreal = C.c_float if sys.argv[3] == "f32" else C.c_double
 class H(C.Structure):
-    _fields_ = [("pauliCodes", C.POINTER(C.c_int)), ("termCoeffs", C.POINTER(C.c_double)),
+    _fields_ = [("pauliCodes", C.POINTER(C.c_int)), ("termCoeffs", C.POINTER(qreal)),
                 ("numSumTerms", C.c_int), ("numQubits", C.c_int)]
 lib.createPauliHamilFromFile.restype = H
 lib.createPauliHamilFromFile.argtypes = [C.c_char_p]
@@ -86,8 +87,12 @@ print(json.dumps({"n": h.numQubits, "terms": h.numSumTerms,
 """
 
 
-def reference_read(path):
-    r = subprocess.run([sys.executable, "-c", _CHILD, REFLIB, path], capture_output=True, text=True, timeout=60)
+REFLIB_F32 = os.path.join(ROOT, "oracle", "_ref", "libquest_ref_f32.so")
+
+
+def reference_read(path, flavour="f64"):
+    ref = REFLIB_F32 if flavour == "f32" else REFLIB
+    r = subprocess.run([sys.executable, "-c", _CHILD, ref, path, flavour], capture_output=True, text=True, timeout=60)
     if r.returncode == 0:
         return json.loads(r.stdout)
     line = [l for l in r.stderr.splitlines() if "QuEST reference error" in l]
@@ -95,12 +100,18 @@ def reference_read(path):
     return {"error": line[0].split(": ", 1)[1]}
 
 
-def library_read(path):
-    import quantumlab_amd
-    from quantumlab_amd.quest import PauliHamil
-    lib = C.CDLL(quantumlab_amd.library_path(), mode=os.RTLD_LOCAL)
+def library_read(path, flavour="f64"):
+    from quantumlab_amd.quest import library_path
+
+    class PauliHamil(C.Structure):
+        _fields_ = [("pauliCodes", C.POINTER(C.c_int)),
+                    ("termCoeffs", C.POINTER(C.c_float if flavour == "f32" else C.c_double)),
+                    ("numSumTerms", C.c_int), ("numQubits", C.c_int)]
+
+    lib = C.CDLL(library_path(flavour), mode=os.RTLD_LOCAL)
     lib.qmxSetErrorMode(1)
     lib.createPauliHamilFromFile.restype = PauliHamil
+    lib.destroyPauliHamil.argtypes = [PauliHamil]
     lib.createPauliHamilFromFile.argtypes = [C.c_char_p]
     lib.qmxLastError.argtypes = [C.c_char_p, C.c_int]
     lib.qmxClearError()
@@ -128,10 +139,11 @@ def test_pauli_hamil_file_matches_reference_reader(tmp_path, name, text):
     assert _same(got, ref), (ref, got)
 
 
-def test_random_pauli_hamil_files_match_reference_reader(tmp_path):
-    """80 seeded random files: the library reads each as QuEST's CPU build does
-    (same values, or the same message)."""
-    if not os.path.exists(REFLIB):
+@pytest.mark.parametrize("flavour", ["f64", "f32"])
+def test_random_pauli_hamil_files_match_reference_reader(tmp_path, flavour):
+    """80 seeded random files: the library reads each as QuEST's CPU build at
+    the same precision does (same values, or the same message)."""
+    if not os.path.exists(REFLIB_F32 if flavour == "f32" else REFLIB):
         pytest.skip("reference QuEST build (oracle/_ref) not present")
     import numpy as np
 
@@ -141,7 +153,7 @@ def test_random_pauli_hamil_files_match_reference_reader(tmp_path):
         path = str(tmp_path / f"h{i}.txt")
         with open(path, "w", newline="") as f:
             f.write(text)
-        ref, got = reference_read(path), library_read(path)
+        ref, got = reference_read(path, flavour), library_read(path, flavour)
         assert _same(got, ref), (repr(text), ref, got)
 
 
